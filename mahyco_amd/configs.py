@@ -1,0 +1,113 @@
+"""Named workloads: the reference's C1 fixture and the scaled configurations.
+
+C1  Sod X, 200x5, one material, VNR + minmod order 2 (reference test
+    tests/SOD_X_Mono_mat_Minmod_vnr_o2, read from its donnees.txt).
+C2  two-material Riemann problem, 2048^2, perturbed interface, superBee +
+    PenteBorne (arithmeticG on pure cells), 200 steps.
+C3  point explosion, 4096^2, one material, minmod order 2, 100 steps.
+C4  three-material layout, 3072x1024, 500 steps.
+"""
+from .abi import MhyConfig, LIM, CASE, EOS_PERFECT_GAS, SCHEMA_VNR
+
+
+def default_config():
+    c = MhyConfig()
+    c.test_case = -1
+    c.nx, c.ny = 30, 30
+    c.dx, c.dy = 1.2 / 30, 1.2 / 30
+    c.final_time, c.output_time = 1.0, 1.0
+    c.deltat_init, c.deltat_min = 0.0, 1.0e-10
+    c.max_time_iterations = 500000000
+    c.nbmat = -1
+    for k in range(3):
+        c.eos[k] = EOS_PERFECT_GAS
+        c.gammap[k] = 1.4
+    c.gamma = 1.4
+    c.schema = -1
+    c.space_order = 2
+    c.avec_projection = 1
+    c.projection_conservative = 0
+    c.threshold = 1.0e-16
+    c.projection_order = 2
+    c.limiter = -1
+    c.limiter_pure = -1
+    c.pp = c.pp_mixte = c.pp_complet = 0
+    c.pseudo_centree = 1
+    c.methode_flux_masse = 0
+    c.sans_lagrange = 0
+    c.abort_on_energy = 1
+    c.precision = 0
+    c.seed = 0
+    return c
+
+
+def _vnr(c):
+    c.schema = SCHEMA_VNR
+    c.avec_projection = 1
+    c.pseudo_centree = 0
+    c.abort_on_energy = 0
+    c.final_time = 1.0e30
+    c.output_time = 1.0e30
+    return c
+
+
+def riemann_perturbed(n=2048, seed=20261202, mixte=0):
+    """C2: two-material Riemann problem on [0,1]^2 with a perturbed interface."""
+    c = _vnr(default_config())
+    c.test_case = CASE["RiemannPerturbed"]
+    c.nx = c.ny = n
+    c.dx = c.dy = 1.0 / n
+    c.nbmat = 2
+    c.gammap[0], c.gammap[1] = 1.4, 1.67
+    c.limiter = LIM["superBee"]
+    c.pp = 1
+    c.pp_mixte = mixte
+    c.limiter_pure = LIM["arithmeticG"]
+    c.pp_complet = 0
+    c.methode_flux_masse = 0
+    c.seed = seed
+    for i, v in enumerate([0.5, 0.01, 8, 0.0, 1.0, 1.0, 0.125, 0.1]):
+        c.param[i] = v
+    return c
+
+
+def point_explosion(n=4096):
+    """C3: one material, energy 0.979 in the 2x2 cells at the origin."""
+    c = _vnr(default_config())
+    c.test_case = CASE["PointExplosion"]
+    c.nx = c.ny = n
+    c.dx = c.dy = 1.0 / n
+    c.nbmat = 1
+    c.limiter = LIM["minmod"]
+    for i, v in enumerate([1.0, 1.0e-6, 0.979]):
+        c.param[i] = v
+    return c
+
+
+def triple_layout(nx=3072, ny=1024, seed=20261203):
+    """C4: three materials on [0,3]x[0,1]."""
+    c = _vnr(default_config())
+    c.test_case = CASE["TripleLayout"]
+    c.nx, c.ny = nx, ny
+    c.dx = 3.0 / nx
+    c.dy = 1.0 / ny
+    c.nbmat = 3
+    c.gammap[0], c.gammap[1], c.gammap[2] = 1.5, 1.4, 1.5
+    c.limiter = LIM["minmod"]
+    c.seed = seed
+    c.param[0] = 1.0e-3
+    return c
+
+
+def riemann(n, x0, rho_l, p_l, rho_r, p_r, g_l, g_r):
+    """One member of the ensemble: straight interface at x = x0."""
+    c = _vnr(default_config())
+    c.test_case = CASE["Riemann"]
+    c.nx = c.ny = n
+    c.dx = c.dy = 1.0 / n
+    c.nbmat = 2
+    c.gammap[0], c.gammap[1] = g_l, g_r
+    c.limiter = LIM["minmod"]
+    for i, v in enumerate([x0, rho_l, p_l, rho_r, p_r]):
+        c.param[i] = v
+    return c

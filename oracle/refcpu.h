@@ -1,0 +1,41 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * C API of the CPU restatement of MaHyCo's VNR Lagrange + directional remap
+ * time loop (oracle/refcpu.cc).  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it, as the checker.
+ */
+#ifndef MAHYCO_ORACLE_REFCPU_H
+#define MAHYCO_ORACLE_REFCPU_H
+
+#include "../include/mahyco.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ref_ctx ref_ctx;
+
+/* Positional donnees.txt reader (LectureDonnees.cc:12-226). */
+int ref_read_donnees(const char* path, mhy_config* cfg);
+void ref_config_default(mhy_config* cfg);
+
+/* Mesh + initial state + setUpTimeLoopN (VnrRemap.cc:330-341). */
+ref_ctx* ref_create(const mhy_config* cfg, int* status);
+void ref_destroy(ref_ctx* r);
+
+/* One loop iteration (VnrRemap.cc:130-222) with the n/n+1 swap, no output.
+ * Returns MHY_OK or an error status. */
+int ref_step(ref_ctx* r);
+/* Whole reference loop with VTP/PVD output into output_dir (NULL/"none":
+ * disabled).  Returns MHY_OK or an error status. */
+int ref_simulate(ref_ctx* r, const char* output_dir, int max_steps);
+
+int ref_get_field(ref_ctx* r, int field, double* out, long long n);
+int ref_get_time_state(ref_ctx* r, mhy_time_state* ts);
+int ref_set_threads(int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

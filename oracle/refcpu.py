@@ -1,0 +1,111 @@
+"""ORACLE -- test infrastructure, not product code.
+
+ctypes binding of the CPU restatement of MaHyCo's VNR Lagrange + remap loop
+(oracle/refcpu.cc, built into oracle/build/librefcpu.so by oracle/Makefile).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from mahyco_amd.abi import MhyConfig, MhyTimeState, MHY_OK, CELL_FIELDS, MAT_FIELDS, NODE_FIELDS, mat_field
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "librefcpu.so")
+RUNNER = os.path.join(HERE, "build", "refcpu_run")
+_lib = None
+
+
+def build(quiet=True):
+    """Compiles the oracle with its Makefile (g++, -ffp-contract=off)."""
+    out = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.ref_read_donnees.argtypes = [ctypes.c_char_p, ctypes.POINTER(MhyConfig)]
+        L.ref_config_default.argtypes = [ctypes.POINTER(MhyConfig)]
+        L.ref_create.argtypes = [ctypes.POINTER(MhyConfig), ctypes.POINTER(ctypes.c_int)]
+        L.ref_create.restype = ctypes.c_void_p
+        L.ref_destroy.argtypes = [ctypes.c_void_p]
+        L.ref_step.argtypes = [ctypes.c_void_p]
+        L.ref_simulate.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+        L.ref_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.c_longlong]
+        L.ref_get_time_state.argtypes = [ctypes.c_void_p, ctypes.POINTER(MhyTimeState)]
+        L.ref_set_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def read_donnees(path):
+    cfg = MhyConfig()
+    st = lib().ref_read_donnees(path.encode(), ctypes.byref(cfg))
+    if st != MHY_OK:
+        raise RuntimeError(f"cannot read {path}: {st}")
+    return cfg
+
+
+def set_threads(n):
+    return lib().ref_set_threads(n)
+
+
+class Oracle:
+    """One problem on the CPU oracle."""
+
+    def __init__(self, cfg):
+        st = ctypes.c_int(0)
+        self.cfg = cfg
+        self.h = lib().ref_create(ctypes.byref(cfg), ctypes.byref(st))
+        if not self.h:
+            raise RuntimeError(f"oracle init failed: {st.value}")
+        self.nx, self.ny = cfg.nx, cfg.ny
+
+    def close(self):
+        if self.h:
+            lib().ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n=1):
+        for _ in range(n):
+            st = lib().ref_step(self.h)
+            if st != MHY_OK:
+                return st
+        return MHY_OK
+
+    def simulate(self, outdir=None, max_steps=0):
+        return lib().ref_simulate(self.h, (outdir or "none").encode(), max_steps)
+
+    def time_state(self):
+        ts = MhyTimeState()
+        lib().ref_get_time_state(self.h, ctypes.byref(ts))
+        return ts
+
+    def field_id(self, name, k=0):
+        if name in CELL_FIELDS:
+            return CELL_FIELDS[name], self.nx * self.ny
+        if name in MAT_FIELDS:
+            return mat_field(MAT_FIELDS[name], k), self.nx * self.ny
+        return NODE_FIELDS[name], (self.nx + 1) * (self.ny + 1)
+
+    def get(self, name, k=0):
+        fid, n = self.field_id(name, k)
+        out = np.empty(n, dtype=np.float64)
+        st = lib().ref_get_field(self.h, fid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+        if st != MHY_OK:
+            raise RuntimeError(f"get {name}: {st}")
+        return out
