@@ -1,0 +1,142 @@
+"""Throughput of the VNR Lagrange + remap step on one MI355X.
+
+A step is one full iteration of the reference time loop (CFL reduction,
+Lagrange phase, two remap sweeps with the dual remap, final remap + EOS)
+over the whole mesh; the metric is cell-updates/s = cells x steps / wall
+time of the timed steps, with the state resident in HBM.
+
+Default workload: the two-material Riemann problem on 2048^2 with a
+perturbed interface, superBee + PenteBorne (configuration C2).
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s (Lagrange+remap) + HBM GB/s fraction; mixed/batched vs plain FP64"
+HBM_PEAK = 8.0e12
+
+
+def workload(name):
+    from mahyco_amd import configs
+    if name == "c2":
+        return "two-material Riemann 2048x2048, perturbed interface, superBee+PenteBorne, FP64", configs.riemann_perturbed()
+    if name == "c3":
+        return "point explosion 4096x4096, 1 material, minmod order 2, FP64", configs.point_explosion()
+    if name == "c4":
+        return "three-material layout 3072x1024, minmod, FP64", configs.triple_layout()
+    if name.startswith("c2@"):
+        n = int(name[3:])
+        c = configs.riemann_perturbed(n=n)
+        return f"two-material Riemann {n}x{n}, perturbed interface, superBee+PenteBorne, FP64", c
+    raise SystemExit(f"unknown config {name}")
+
+
+def algorithmic_bytes_per_cell(m):
+    # compulsory persistent state per cell-update, read once and written once:
+    # cell rho,p,q,c,divu + 7 words per material, mixture e written, node
+    # u_x,u_y,M, mixed/pure flags (4 B read + 4 B written each)
+    return 8 * (17 + 14 * m) + 16
+
+
+def cpu_baseline(cfg, seconds_budget):
+    """The CPU oracle (same arithmetic, OpenMP over cells) on the same mesh
+    for a bounded number of steps."""
+    from oracle import refcpu
+    cores = os.cpu_count() or 1
+    refcpu.set_threads(cores)
+    orc = refcpu.Oracle(cfg)
+    orc.step(1)  # warm-up step (first-touch of the arrays)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        orc.step(1)
+        n += 1
+        if time.perf_counter() - t0 > seconds_budget or n >= 20:
+            break
+    dt = time.perf_counter() - t0
+    orc.close()
+    cells = cfg.nx * cfg.ny
+    return {"value": cells * n / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"CPU oracle (reference-semantics restatement, g++ -O2 -fopenmp), same {cfg.nx}x{cfg.ny} "
+                      f"mesh, {n} timed steps after 1 warm-up step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import mahyco_amd
+
+    desc, cfg = workload(args.config)
+    ctx = mahyco_amd.Context(cfg)
+    cells = cfg.nx * cfg.ny
+    ctx.run(args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(args.steps)  # ends with a stream synchronisation
+    wall = time.perf_counter() - t0
+    value = cells * args.steps / wall
+
+    # per-kernel durations from HIP events on the context stream
+    ctx.enable_timing(True)
+    ctx.run(max(3, min(args.steps, 10)))
+    stats = ctx.kernel_stats()
+    ctx.enable_timing(False)
+    ctx.close()
+    tot = {k: v["avg_ms"] * v["launches"] for k, v in stats.items()}
+    dom = max(tot, key=tot.get)
+    dk = stats[dom]
+    achieved = dk["bytes"] / (dk["avg_ms"] * 1e-3) if dk["avg_ms"] > 0 else 0.0
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        try:
+            with open(args.traffic_json) as fh:
+                traffic = json.load(fh).get(args.config, {}).get(dom)
+        except (OSError, ValueError):
+            traffic = None
+    B = algorithmic_bytes_per_cell(cfg.nbmat)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "mesh": [cfg.nx, cfg.ny], "nbmat": cfg.nbmat, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},
+        "step_roofline": {"algorithmic_bytes_per_cell_update": B, "achieved": B * value / 1e9,
+                          "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B * value / HBM_PEAK},
+        "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "GBps_alg": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)
+                        if v["avg_ms"] > 0 else 0} for k, v in stats.items() if v["launches"]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

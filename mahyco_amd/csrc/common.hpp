@@ -1,0 +1,134 @@
+// Shared host/device definitions of the VNR Lagrange + remap hot path:
+// floating-point helpers with the exact semantics the reference relies on,
+// the device state layout and the per-launch parameter block.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "../../include/mahyco.h"
+
+#define MHY_HD __host__ __device__ __forceinline__
+
+namespace mhy {
+
+constexpr int NMAT = MHY_NBMATMAX;
+// live remap components: volume, mass, energy per material, then the
+// pseudo-viscosity content (reference vector index 3m+3; indices 3m..3m+2
+// carry momentum and kinetic energy, always zero in VNR cell vectors)
+constexpr int NCMAX = 3 * NMAT + 1;
+
+// std::min / std::max argument conventions (cppnabla/types/MathFunctions.h:23-24)
+MHY_HD double smin(double a, double b) { return (b < a) ? b : a; }
+MHY_HD double smax(double a, double b) { return (a < b) ? b : a; }
+// includes/Freefunctions.h:86-92
+MHY_HD double divideNoExcept(double a, double b) { return (fabs(b) < 1.0E-12) ? 0.0 : a / b; }
+
+struct D2 {
+  double x, y;
+};
+MHY_HD D2 d2add(D2 a, D2 b) { return {a.x + b.x, a.y + b.y}; }
+MHY_HD D2 d2sub(D2 a, D2 b) { return {a.x - b.x, a.y - b.y}; }
+MHY_HD D2 d2mul(D2 a, double s) { return {a.x * s, a.y * s}; }
+MHY_HD D2 d2div(D2 a, double s) { return {a.x / s, a.y / s}; }
+// dot product accumulated from 0.0 (includes/Freefunctions.h:13-23)
+MHY_HD double d2dot(D2 a, D2 b) {
+  double r = 0.0;
+  r = r + a.x * b.x;
+  r = r + a.y * b.y;
+  return r;
+}
+MHY_HD double d2norm(D2 a) { return sqrt(d2dot(a, a)); }
+MHY_HD double d2cross(D2 a, D2 b) { return a.x * b.y - a.y * b.x; }
+
+// corner normal |C|.(C/|C|) per half edge (includes/Freefunctions.h:25-58)
+MHY_HD D2 lpcNpc(D2 xp, D2 xpPlus, D2 xpMinus) {
+  D2 n1 = {0.5 * (xpPlus.y - xp.y), 0.5 * (xp.x - xpPlus.x)};
+  double l1 = d2norm(n1);
+  D2 n2 = {0.5 * (xp.y - xpMinus.y), 0.5 * (xpMinus.x - xp.x)};
+  double l2 = d2norm(n2);
+  return d2add(d2mul(d2div(n1, l1), l1), d2mul(d2div(n2, l2), l2));
+}
+// includes/Freefunctions.h:60-65
+MHY_HD D2 symmetricVector(D2 v, D2 sigma) { return d2sub(d2mul(sigma, 2.0 * d2dot(v, sigma)), v); }
+
+// sub-volume of corner s of a quad (vnr_remap/PhaseLagrange.cc:236-264)
+MHY_HD double subVolume(D2 xc, D2 xm, D2 x0, D2 xp) {
+  const D2 x2 = d2mul(d2add(xm, x0), 0.5);
+  const D2 x4 = d2mul(d2add(xp, x0), 0.5);
+  return 0.5 * (d2cross(xc, x2) + d2cross(x2, x0) + d2cross(x0, x4) + d2cross(x4, xc));
+}
+
+// Run mode of a context.
+enum Mode : int {
+  MODE_FULL = 0,       // Lagrange + remap (Euler grid fixed)
+  MODE_LAGRANGE = 1,   // AvecProjection = 0: moving mesh
+  MODE_ADVECTION = 2,  // sansLagrange = 1: remap of an imposed velocity field
+};
+
+// Time-loop scalars, double-buffered like the reference's n/n+1 pairs.
+struct Scalars {
+  double t[2];
+  double dt[2];
+  double pmax;
+  int err;        // MHY_ERR_* raised by a kernel (energy abort, NaN)
+  unsigned int done_blocks;
+  int pad;
+};
+
+// Options and geometry constants, uniform for a launch.
+struct Opts {
+  int nx, ny, nbmat, nc;  // nc = 3*nbmat + 1 live remap components
+  int mode;
+  int order;              // projectionOrder
+  int limiter, limiter_pure, pp, pp_mixte, pp_complet;
+  int pseudo_centree, methode, abort_energy;
+  int eos[NMAT];
+  double gammap[NMAT];
+  double gamma, threshold, dx, dy, cfl, cfl_euler;
+  // boundary conditions (includes/ConditionsLimites.h:9-43)
+  int bc_left, bc_right, bc_bottom, bc_top;
+  double bcv_left[2], bcv_right[2], bcv_bottom[2], bcv_top[2];
+  int periodic_right;
+  int flux_left, flux_right, flux_bottom, flux_top, flux_any;
+};
+
+// Device pointers of one problem (structure of arrays, FP64, cell index
+// j*nx+i, node index j*(nx+1)+i).  [2] arrays are the reference's n/n+1
+// pairs, addressed through the parity of the launch.
+struct Fields {
+  double *rho[2], *p[2], *q[2], *c[2], *divu[2], *e[2];
+  double *rho_k[2][NMAT], *p_k[2][NMAT], *q_k[2][NMAT], *c_k[2][NMAT], *e_k[2][NMAT];
+  double *ux[2], *uy[2];
+  double *X[2], *Y[2];  // moving node coordinates (MODE_LAGRANGE)
+  double *vol[2];       // sum of the corner sub-volumes of the step start
+  double *f_k[NMAT], *mu_k[NMAT], *M, *M_k[NMAT];
+  int *mixte, *pure;
+  double *Mp, *outvx, *outvy;
+  double *rho0, *e0, *rho_k0[NMAT], *e_k0[NMAT];  // t=0 fields (periodic cells)
+  // per-step intermediates
+  double *UL[NCMAX], *U1[NCMAX], *U2[NCMAX];
+  double *VL, *XcL, *YcL, *rL, *qL;
+  double *G[NMAT];   // face fluxes of the current sweep for the dual remap (A1/A2)
+  double *DF[NMAT];  // PB dual flux per cell of the current sweep
+  double *UD1[3];    // nodal vector after dual sweep 1 (M ux, M uy, M)
+  Scalars* sc;
+  double* partials;  // per-block reduction partials (2 per block)
+};
+
+// Parameters of one launch: fields, options, parity and sweep direction.
+struct KP {
+  Fields f;
+  Opts o;
+  int cur, nxt;  // parity of the n and n+1 buffers
+  int sweep;     // 1 or 2
+  int xdir;      // 1: sweep along x, 0: along y
+  long ncell, nnode;
+};
+
+// Euler node coordinates (CartesianMesh2DGenerator.cc:52-55: x = dx*i, y = dy*j)
+MHY_HD D2 eulerNode(const Opts& o, long j, long i) { return {o.dx * (double)i, o.dy * (double)j}; }
+
+}  // namespace mhy

@@ -1,0 +1,717 @@
+// Context of one problem on one GPU: device allocation and upload of the
+// initial state, the per-step kernel sequence on the context stream, field
+// access, conservation sums, kernel timing, and the extern "C" ABI declared
+// in include/mahyco.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "host/init.hpp"
+#include "host/output.hpp"
+#include "kernels.hpp"
+
+using namespace mhy;
+
+namespace {
+
+enum KernelId { K_DT = 0, K_NODE, K_CELL, K_SWEEP_X, K_SWEEP_Y, K_DUAL_X, K_DUAL_Y, K_FINAL, K_COUNT };
+const char* kKernelNames[K_COUNT] = {"dt_reduce", "lagrange_node", "lagrange_cell", "sweep_x",
+                                     "sweep_y",   "dual_x",        "dual_y",        "remap_final"};
+
+constexpr int kBlock = 256;
+constexpr int kReduceBlocks = 1024;
+
+}  // namespace
+
+struct mhy_ctx {
+  mhy_config cfg;
+  Opts o;
+  Fields f;
+  HostInit init;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int cur = 0;
+  bool x_then_y_n = true;
+  int iteration = 0;
+  bool finished = false;
+  int status = MHY_OK;
+  long ncell = 0, nnode = 0, nface = 0;
+  std::string err;
+  std::vector<void*> allocs;
+  // kernel timing
+  bool timing = false;
+  struct Rec {
+    int k;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> eventPool;
+  double ms[K_COUNT] = {0};
+  int launches[K_COUNT] = {0};
+  // output
+  OutputWriter* writer = nullptr;
+};
+
+namespace {
+
+int fail(mhy_ctx* x, int code, const std::string& msg) {
+  if (x) {
+    x->err = msg;
+    x->status = code;
+  }
+  return code;
+}
+
+#define HIPCHK(x, call)                                                                             \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess) return fail((x), MHY_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Allocation helpers keep the first error in e and skip later requests.
+double* allocD(mhy_ctx* x, long n, hipError_t& e) {
+  if (e != hipSuccess) return nullptr;
+  void* p = nullptr;
+  e = hipMalloc(&p, sizeof(double) * (size_t)std::max(n, 1L));
+  if (e == hipSuccess) {
+    x->allocs.push_back(p);
+    e = hipMemset(p, 0, sizeof(double) * (size_t)std::max(n, 1L));
+  }
+  return static_cast<double*>(p);
+}
+
+int* allocI(mhy_ctx* x, long n, hipError_t& e) {
+  if (e != hipSuccess) return nullptr;
+  void* p = nullptr;
+  e = hipMalloc(&p, sizeof(int) * (size_t)std::max(n, 1L));
+  if (e == hipSuccess) {
+    x->allocs.push_back(p);
+    e = hipMemset(p, 0, sizeof(int) * (size_t)std::max(n, 1L));
+  }
+  return static_cast<int*>(p);
+}
+
+KP makeKP(const mhy_ctx* x) {
+  KP kp;
+  kp.f = x->f;
+  kp.o = x->o;
+  kp.cur = x->cur;
+  kp.nxt = 1 - x->cur;
+  kp.sweep = 1;
+  kp.xdir = 1;
+  kp.ncell = x->ncell;
+  kp.nnode = x->nnode;
+  return kp;
+}
+
+hipEvent_t takeEvent(mhy_ctx* x) {
+  if (!x->eventPool.empty()) {
+    hipEvent_t e = x->eventPool.back();
+    x->eventPool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Launch helper: optional HIP events around the kernel on the context stream.
+template <class F>
+void timed(mhy_ctx* x, int k, F&& launch) {
+  if (!x->timing) {
+    launch();
+    return;
+  }
+  mhy_ctx::Rec r;
+  r.k = k;
+  r.a = takeEvent(x);
+  r.b = takeEvent(x);
+  (void)hipEventRecord(r.a, x->stream);
+  launch();
+  (void)hipEventRecord(r.b, x->stream);
+  x->pending.push_back(r);
+}
+
+void collectTimes(mhy_ctx* x) {
+  for (auto& r : x->pending) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+      x->ms[r.k] += t;
+      x->launches[r.k] += 1;
+    }
+    x->eventPool.push_back(r.a);
+    x->eventPool.push_back(r.b);
+  }
+  x->pending.clear();
+}
+
+dim3 gridFor(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+int checkLaunch(mhy_ctx* x, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(x, MHY_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  return MHY_OK;
+}
+
+int phaseDt(mhy_ctx* x) {
+  KP kp = makeKP(x);
+  if (x->o.mode == MODE_ADVECTION) {
+    timed(x, K_DT, [&] { hipLaunchKernelGGL(k_dt_hold, dim3(1), dim3(64), 0, x->stream, kp); });
+  } else {
+    const unsigned nb = (unsigned)std::min<long>(kReduceBlocks, (x->ncell + kBlock - 1) / kBlock);
+    timed(x, K_DT, [&] { hipLaunchKernelGGL(k_dt_reduce, dim3(nb), dim3(kBlock), 0, x->stream, kp); });
+  }
+  return checkLaunch(x, "dt");
+}
+
+int phaseLagrange(mhy_ctx* x) {
+  KP kp = makeKP(x);
+  timed(x, K_NODE, [&] { hipLaunchKernelGGL(k_lagrange_node, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
+  timed(x, K_CELL, [&] { hipLaunchKernelGGL(k_lagrange_cell, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+  return checkLaunch(x, "lagrange");
+}
+
+int phaseSweep(mhy_ctx* x, int sweep) {
+  KP kp = makeKP(x);
+  kp.sweep = sweep;
+  const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
+  kp.xdir = xd ? 1 : 0;
+  if (xd) {
+    timed(x, K_SWEEP_X,
+          [&] { hipLaunchKernelGGL(k_sweep<true>, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+    timed(x, K_DUAL_X, [&] { hipLaunchKernelGGL(k_dual<true>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
+  } else {
+    timed(x, K_SWEEP_Y,
+          [&] { hipLaunchKernelGGL(k_sweep<false>, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+    timed(x, K_DUAL_Y,
+          [&] { hipLaunchKernelGGL(k_dual<false>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
+  }
+  return checkLaunch(x, "sweep");
+}
+
+int phaseFinal(mhy_ctx* x) {
+  KP kp = makeKP(x);
+  timed(x, K_FINAL, [&] { hipLaunchKernelGGL(k_remap_final, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+  return checkLaunch(x, "final");
+}
+
+int readScalars(mhy_ctx* x, Scalars& s) {
+  HIPCHK(x, hipMemcpyAsync(&s, x->f.sc, sizeof(Scalars), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  collectTimes(x);
+  return MHY_OK;
+}
+
+// Swap of the n / n+1 buffers (VnrRemap.cc:198-222) = parity flip.
+void endIteration(mhy_ctx* x) {
+  x->cur = 1 - x->cur;
+  if (x->o.mode != MODE_LAGRANGE) x->x_then_y_n = !x->x_then_y_n;
+}
+
+int runIteration(mhy_ctx* x) {
+  int st;
+  if ((st = phaseDt(x)) != MHY_OK) return st;
+  if ((st = phaseLagrange(x)) != MHY_OK) return st;
+  if (x->o.mode != MODE_LAGRANGE) {
+    if ((st = phaseSweep(x, 1)) != MHY_OK) return st;
+    if ((st = phaseSweep(x, 2)) != MHY_OK) return st;
+    if ((st = phaseFinal(x)) != MHY_OK) return st;
+  }
+  x->iteration++;
+  return MHY_OK;
+}
+
+int upload(mhy_ctx* x, double* dst, const std::vector<double>& src) {
+  HIPCHK(x, hipMemcpy(dst, src.data(), sizeof(double) * src.size(), hipMemcpyHostToDevice));
+  return MHY_OK;
+}
+
+// device pointer and length of a field id (state n of the reference)
+double* fieldPtr(mhy_ctx* x, int field, long& n) {
+  const int c = x->cur;
+  n = x->ncell;
+  switch (field) {
+    case MHY_CELL_DENSITY: return x->f.rho[c];
+    case MHY_CELL_PRESSURE: return x->f.p[c];
+    case MHY_CELL_ENERGY: return x->f.e[c];
+    case MHY_CELL_PSEUDO: return x->f.q[c];
+    case MHY_CELL_SOUND_SPEED: return x->f.c[c];
+    case MHY_CELL_DIVU: return x->f.divu[c];
+    case MHY_CELL_MASS: return x->f.M;
+    default: break;
+  }
+  if (field >= MHY_MAT_FRACVOL && field < MHY_MAT_FRACVOL + MHY_NBMATMAX * MHY_MAT_STRIDE) {
+    const int k = (field - MHY_MAT_FRACVOL) / MHY_MAT_STRIDE, id = field - k * MHY_MAT_STRIDE;
+    if (k >= x->o.nbmat) return nullptr;
+    switch (id) {
+      case MHY_MAT_FRACVOL: return x->f.f_k[k];
+      case MHY_MAT_MASS_FRACTION: return x->f.mu_k[k];
+      case MHY_MAT_DENSITY: return x->f.rho_k[c][k];
+      case MHY_MAT_ENERGY: return x->f.e_k[c][k];
+      case MHY_MAT_PRESSURE: return x->f.p_k[c][k];
+      case MHY_MAT_PSEUDO: return x->f.q_k[c][k];
+      case MHY_MAT_MASS: return x->f.M_k[k];
+      case MHY_MAT_SOUND_SPEED: return x->f.c_k[c][k];
+      default: return nullptr;
+    }
+  }
+  n = x->nnode;
+  switch (field) {
+    case MHY_NODE_VELOCITY_X: return x->f.ux[c];
+    case MHY_NODE_VELOCITY_Y: return x->f.uy[c];
+    case MHY_NODE_MASS: return x->f.Mp;
+    case MHY_NODE_COORD_X: return x->o.mode == MODE_LAGRANGE ? x->f.X[c] : nullptr;
+    case MHY_NODE_COORD_Y: return x->o.mode == MODE_LAGRANGE ? x->f.Y[c] : nullptr;
+    case MHY_NODE_OUT_VX: return x->o.mode == MODE_LAGRANGE ? x->f.outvx : x->f.ux[c];
+    case MHY_NODE_OUT_VY: return x->o.mode == MODE_LAGRANGE ? x->f.outvy : x->f.uy[c];
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- C ABI
+
+extern "C" {
+
+const char* mhy_last_error(const mhy_ctx* x) { return x ? x->err.c_str() : "null context"; }
+
+int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
+  if (!cfg || !out) return MHY_ERR_ARG;
+  *out = nullptr;
+  mhy_ctx* x = new mhy_ctx();
+  x->cfg = *cfg;
+  const mhy_config& c = *cfg;
+  auto bail = [&](int code) {
+    // keep the context so the caller can read the message, unless it is
+    // unusable; mhy_destroy releases it
+    *out = x;
+    return code;
+  };
+  if (c.schema != MHY_SCHEMA_VNR) return bail(fail(x, MHY_ERR_ARG, "only the VNR scheme is supported"));
+  if (c.projection_order > 2) return bail(fail(x, MHY_ERR_ARG, "projection order 3 is not supported"));
+  if (c.precision != 0) return bail(fail(x, MHY_ERR_ARG, "only FP64 (precision 0) is supported"));
+  if (c.sans_lagrange && !c.avec_projection)
+    return bail(fail(x, MHY_ERR_ARG, "sansLagrange needs the projection"));
+  std::string err;
+  int st = buildInitialState(c, x->init, err);
+  if (st != MHY_OK) return bail(fail(x, st, err));
+  if (c.avec_projection && !outerNormalsAxisExact(c, err)) return bail(fail(x, MHY_ERR_ARG, err));
+
+  const HostInit& h = x->init;
+  x->ncell = (long)c.nx * c.ny;
+  x->nnode = (long)(c.nx + 1) * (c.ny + 1);
+  const long C = x->ncell, N = x->nnode;
+  const int m = c.nbmat;
+  Opts& o = x->o;
+  std::memset(&o, 0, sizeof(o));
+  o.nx = c.nx, o.ny = c.ny, o.nbmat = m, o.nc = 3 * m + 1;
+  o.mode = c.sans_lagrange ? MODE_ADVECTION : (c.avec_projection ? MODE_FULL : MODE_LAGRANGE);
+  o.order = c.projection_order;
+  o.limiter = c.limiter, o.limiter_pure = c.limiter_pure;
+  o.pp = c.pp, o.pp_mixte = c.pp_mixte, o.pp_complet = c.pp_complet;
+  o.pseudo_centree = c.pseudo_centree, o.methode = c.methode_flux_masse;
+  o.abort_energy = c.abort_on_energy;
+  for (int k = 0; k < MHY_NBMATMAX; ++k) o.eos[k] = c.eos[k], o.gammap[k] = c.gammap[k];
+  o.gamma = c.gamma, o.threshold = c.threshold, o.dx = c.dx, o.dy = c.dy;
+  // computeDeltaT constants (VnrRemap.cc:16-22)
+  o.cfl = c.avec_projection == 1 ? 0.05 : 0.1;
+  o.cfl_euler = c.avec_projection == 1 ? 1.0 : 0.0;
+  const Boundary& b = h.bc;
+  o.bc_left = b.left, o.bc_right = b.right, o.bc_bottom = b.bottom, o.bc_top = b.top;
+  for (int d = 0; d < 2; ++d)
+    o.bcv_left[d] = b.vleft[d], o.bcv_right[d] = b.vright[d], o.bcv_bottom[d] = b.vbottom[d], o.bcv_top[d] = b.vtop[d];
+  o.periodic_right = b.periodic_right;
+  o.flux_left = b.flux_left, o.flux_right = b.flux_right, o.flux_bottom = b.flux_bottom, o.flux_top = b.flux_top;
+  o.flux_any = (b.flux_left + b.flux_right + b.flux_bottom + b.flux_top) > 0;
+
+  hipError_t e = hipSuccess;
+  if ((e = hipGetDevice(&x->device)) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, hipGetErrorString(e)));
+  if ((e = hipStreamCreate(&x->stream)) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, hipGetErrorString(e)));
+  Fields& f = x->f;
+  std::memset(&f, 0, sizeof(f));
+#define AD(ptr, n) ((ptr) = allocD(x, (n), e))
+  for (int s = 0; s < 2; ++s) {
+    AD(f.rho[s], C), AD(f.p[s], C), AD(f.q[s], C), AD(f.c[s], C), AD(f.divu[s], C), AD(f.e[s], C);
+    for (int k = 0; k < m; ++k) {
+      AD(f.rho_k[s][k], C), AD(f.p_k[s][k], C), AD(f.q_k[s][k], C), AD(f.c_k[s][k], C), AD(f.e_k[s][k], C);
+    }
+    AD(f.ux[s], N), AD(f.uy[s], N);
+  }
+  if (o.mode == MODE_LAGRANGE) {
+    for (int s = 0; s < 2; ++s) AD(f.X[s], N), AD(f.Y[s], N), AD(f.vol[s], C);
+    AD(f.outvx, N), AD(f.outvy, N);
+  } else {
+    AD(f.vol[0], C);
+    f.vol[1] = f.vol[0];  // Euler sub-volumes: constant
+  }
+  for (int k = 0; k < m; ++k) AD(f.f_k[k], C), AD(f.mu_k[k], C), AD(f.M_k[k], C);
+  AD(f.M, C);
+  f.mixte = allocI(x, C, e);
+  f.pure = allocI(x, C, e);
+  AD(f.Mp, N);
+  if (o.periodic_right) {
+    AD(f.rho0, C), AD(f.e0, C);
+    for (int k = 0; k < m; ++k) AD(f.rho_k0[k], C), AD(f.e_k0[k], C);
+  }
+  if (o.mode != MODE_LAGRANGE) {
+    for (int q = 0; q < o.nc; ++q) AD(f.UL[q], C), AD(f.U1[q], C), AD(f.U2[q], C);
+    AD(f.VL, C), AD(f.XcL, C), AD(f.YcL, C);
+    if (o.methode == 1) AD(f.rL, C);
+    if (o.pp == 1) AD(f.qL, C);
+    x->nface = std::max((long)(c.nx + 1) * c.ny, (long)c.nx * (c.ny + 1));
+    if (o.methode == 0 || o.methode == 1)
+      for (int k = 0; k < m; ++k) AD(f.G[k], x->nface);
+    if (o.methode == 2)
+      for (int k = 0; k < m; ++k) AD(f.DF[k], C);
+    for (int d = 0; d < 3; ++d) AD(f.UD1[d], N);
+  }
+  if (e == hipSuccess) {
+    void* p = nullptr;
+    if ((e = hipMalloc(&p, sizeof(Scalars))) == hipSuccess) x->allocs.push_back(p);
+    f.sc = static_cast<Scalars*>(p);
+  }
+  AD(f.partials, 2 * kReduceBlocks);
+#undef AD
+  if (e != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, std::string("device allocation: ") + hipGetErrorString(e)));
+
+  // initial state into the "n" slot 0 (setUpTimeLoopN, VnrRemap.cc:68-87)
+  int u = MHY_OK;
+  auto up = [&](double* d, const std::vector<double>& s) {
+    if (u == MHY_OK) u = upload(x, d, s);
+  };
+  up(f.rho[0], h.rho), up(f.p[0], h.p), up(f.c[0], h.c), up(f.e[0], h.e);
+  for (int k = 0; k < m; ++k) {
+    up(f.rho_k[0][k], h.rho_k[k]), up(f.p_k[0][k], h.p_k[k]), up(f.c_k[0][k], h.c_k[k]), up(f.e_k[0][k], h.e_k[k]);
+    up(f.f_k[k], h.f_k[k]), up(f.mu_k[k], h.mu_k[k]), up(f.M_k[k], h.M_k[k]);
+  }
+  up(f.M, h.M);
+  up(f.vol[0], h.vol);
+  up(f.ux[0], h.ux), up(f.uy[0], h.uy);
+  if (o.mode == MODE_LAGRANGE) {
+    std::vector<double> X(N), Y(N);
+    for (long j = 0; j <= c.ny; ++j)
+      for (long i = 0; i <= c.nx; ++i) {
+        const long p = j * (c.nx + 1) + i;
+        X[p] = c.dx * (double)i, Y[p] = c.dy * (double)j;
+      }
+    up(f.X[0], X), up(f.Y[0], Y);
+    up(f.outvx, h.ux), up(f.outvy, h.uy);
+  }
+  if (o.periodic_right) {
+    up(f.rho0, h.rho), up(f.e0, h.e);
+    for (int k = 0; k < m; ++k) up(f.rho_k0[k], h.rho_k[k]), up(f.e_k0[k], h.e_k[k]);
+  }
+  if (u != MHY_OK) return bail(u);
+  if (hipMemcpy(f.mixte, h.mixte.data(), sizeof(int) * C, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(f.pure, h.pure.data(), sizeof(int) * C, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(x, MHY_ERR_DEVICE, "upload flags"));
+  Scalars s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.t[0] = 0.0;
+  s0.dt[0] = h.deltat_init;
+  if (hipMemcpy(f.sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(x, MHY_ERR_DEVICE, "upload scalars"));
+  // computeNodeMass of the prologue (VnrRemap.cc:341)
+  {
+    KP kp = makeKP(x);
+    hipLaunchKernelGGL(k_node_mass, gridFor(N), dim3(kBlock), 0, x->stream, kp);
+    if (hipStreamSynchronize(x->stream) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, "node mass"));
+  }
+  x->cur = 0;
+  x->x_then_y_n = true;
+  x->iteration = 0;
+  *out = x;
+  return MHY_OK;
+}
+
+int mhy_destroy(mhy_ctx* x) {
+  if (!x) return MHY_OK;
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  for (auto& r : x->pending) (void)hipEventDestroy(r.a), (void)hipEventDestroy(r.b);
+  for (auto ev : x->eventPool) (void)hipEventDestroy(ev);
+  for (void* p : x->allocs) (void)hipFree(p);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x->writer;
+  delete x;
+  return MHY_OK;
+}
+
+int mhy_step(mhy_ctx* x, int sync) {
+  if (!x) return MHY_ERR_ARG;
+  if (x->status != MHY_OK) return x->status;
+  int st = runIteration(x);
+  if (st != MHY_OK) return st;
+  endIteration(x);
+  if (sync) {
+    Scalars s;
+    if ((st = readScalars(x, s)) != MHY_OK) return st;
+    if (s.err) return fail(x, s.err, "kernel status " + std::to_string(s.err));
+  }
+  return MHY_OK;
+}
+
+int mhy_run(mhy_ctx* x, int nsteps) {
+  if (!x) return MHY_ERR_ARG;
+  if (x->status != MHY_OK) return x->status;
+  for (int s = 0; s < nsteps; ++s) {
+    int st = runIteration(x);
+    if (st != MHY_OK) return st;
+    endIteration(x);
+  }
+  Scalars s;
+  int st = readScalars(x, s);
+  if (st != MHY_OK) return st;
+  if (s.err) return fail(x, s.err, "kernel status " + std::to_string(s.err));
+  return MHY_OK;
+}
+
+int mhy_compute_dt(mhy_ctx* x) { return x ? phaseDt(x) : MHY_ERR_ARG; }
+int mhy_lagrange(mhy_ctx* x) { return x ? phaseLagrange(x) : MHY_ERR_ARG; }
+// the remap preparation is fused into the cell kernel of mhy_lagrange
+int mhy_prepare_remap(mhy_ctx* x) { return x ? MHY_OK : MHY_ERR_ARG; }
+int mhy_remap_sweep(mhy_ctx* x, int sweep) {
+  if (!x || (sweep != 1 && sweep != 2)) return MHY_ERR_ARG;
+  if (x->o.mode == MODE_LAGRANGE) return MHY_OK;
+  return phaseSweep(x, sweep);
+}
+int mhy_remap_final(mhy_ctx* x) {
+  if (!x) return MHY_ERR_ARG;
+  if (x->o.mode == MODE_LAGRANGE) return MHY_OK;
+  return phaseFinal(x);
+}
+int mhy_end_iteration(mhy_ctx* x) {
+  if (!x) return MHY_ERR_ARG;
+  x->iteration++;
+  endIteration(x);
+  return MHY_OK;
+}
+
+int mhy_synchronize(mhy_ctx* x) {
+  if (!x) return MHY_ERR_ARG;
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  collectTimes(x);
+  return MHY_OK;
+}
+
+int mhy_get_time_state(mhy_ctx* x, mhy_time_state* ts) {
+  if (!x || !ts) return MHY_ERR_ARG;
+  Scalars s;
+  int st = readScalars(x, s);
+  if (st != MHY_OK) return st;
+  ts->t_n = s.t[x->cur], ts->t_nplus1 = s.t[1 - x->cur];
+  ts->deltat_n = s.dt[x->cur], ts->deltat_nplus1 = s.dt[1 - x->cur];
+  ts->iteration = x->iteration;
+  ts->x_then_y_n = x->x_then_y_n ? 1 : 0;
+  ts->finished = x->finished ? 1 : 0;
+  ts->status = s.err ? s.err : x->status;
+  return MHY_OK;
+}
+
+int mhy_get_mesh_size(const mhy_ctx* x, int* nx, int* ny) {
+  if (!x) return MHY_ERR_ARG;
+  if (nx) *nx = x->o.nx;
+  if (ny) *ny = x->o.ny;
+  return MHY_OK;
+}
+
+int mhy_get_field(mhy_ctx* x, int field, double* out, long long n) {
+  if (!x || !out) return MHY_ERR_ARG;
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  if (field == MHY_CELL_MIXTE || field == MHY_CELL_PURE) {
+    if (n != x->ncell) return fail(x, MHY_ERR_ARG, "size");
+    std::vector<int> v(x->ncell);
+    HIPCHK(x, hipMemcpy(v.data(), field == MHY_CELL_MIXTE ? x->f.mixte : x->f.pure, sizeof(int) * x->ncell,
+                        hipMemcpyDeviceToHost));
+    for (long c = 0; c < x->ncell; ++c) out[c] = v[c];
+    return MHY_OK;
+  }
+  if (field == MHY_NODE_MASS) {
+    KP kp = makeKP(x);
+    hipLaunchKernelGGL(k_node_mass, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp);
+    HIPCHK(x, hipStreamSynchronize(x->stream));
+  }
+  if ((field == MHY_NODE_COORD_X || field == MHY_NODE_COORD_Y) && x->o.mode != MODE_LAGRANGE) {
+    if (n != x->nnode) return fail(x, MHY_ERR_ARG, "size");
+    for (long j = 0; j <= x->o.ny; ++j)
+      for (long i = 0; i <= x->o.nx; ++i)
+        out[j * (x->o.nx + 1) + i] = (field == MHY_NODE_COORD_X) ? x->o.dx * (double)i : x->o.dy * (double)j;
+    return MHY_OK;
+  }
+  long len = 0;
+  double* p = fieldPtr(x, field, len);
+  if (!p) return fail(x, MHY_ERR_ARG, "unknown field " + std::to_string(field));
+  if (n != len) return fail(x, MHY_ERR_ARG, "size");
+  HIPCHK(x, hipMemcpy(out, p, sizeof(double) * len, hipMemcpyDeviceToHost));
+  return MHY_OK;
+}
+
+int mhy_set_field(mhy_ctx* x, int field, const double* in, long long n) {
+  if (!x || !in) return MHY_ERR_ARG;
+  long len = 0;
+  double* p = fieldPtr(x, field, len);
+  if (!p || field == MHY_NODE_MASS) return fail(x, MHY_ERR_ARG, "field not settable");
+  if (n != len) return fail(x, MHY_ERR_ARG, "size");
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  HIPCHK(x, hipMemcpy(p, in, sizeof(double) * len, hipMemcpyHostToDevice));
+  return MHY_OK;
+}
+
+int mhy_conservation(mhy_ctx* x, double* sums) {
+  if (!x || !sums) return MHY_ERR_ARG;
+  const int m = x->o.nbmat, nq = 2 + m;
+  const unsigned nb = (unsigned)std::min<long>(256, (x->ncell + kBlock - 1) / kBlock);
+  double* d = nullptr;
+  HIPCHK(x, hipMalloc(&d, sizeof(double) * nb * nq * 4));
+  KP kp = makeKP(x);
+  hipLaunchKernelGGL(k_conservation, dim3(nb), dim3(kBlock), 0, x->stream, kp, d);
+  std::vector<double> h(nb * nq * 4);
+  hipError_t e1 = hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, x->stream);
+  hipError_t e2 = hipStreamSynchronize(x->stream);
+  (void)hipFree(d);
+  if (e1 != hipSuccess || e2 != hipSuccess) return fail(x, MHY_ERR_DEVICE, "conservation");
+  for (int q = 0; q < nq; ++q) {
+    // Neumaier sum of the block sums and compensations
+    double t = 0.0, cc = 0.0;
+    for (unsigned bl = 0; bl < nb; ++bl)
+      for (int part = 0; part < 4; ++part) {
+        const double v = h[(bl * nq + q) * 4 + part];
+        const double y = t + v;
+        if (std::fabs(t) >= std::fabs(v)) cc += (t - y) + v;
+        else cc += (v - y) + t;
+        t = y;
+      }
+    sums[q] = t + cc;
+  }
+  return MHY_OK;
+}
+
+int mhy_kernel_count(const mhy_ctx*) { return K_COUNT; }
+const char* mhy_kernel_name(const mhy_ctx*, int k) { return (k >= 0 && k < K_COUNT) ? kKernelNames[k] : ""; }
+
+// Algorithmic bytes of one launch: every input word the kernel must read
+// from HBM once and every output word it must write once (neighbour reuse
+// through the caches counted once).
+int mhy_kernel_bytes(const mhy_ctx* x, int k, double* bytes) {
+  if (!x || !bytes || k < 0 || k >= K_COUNT) return MHY_ERR_ARG;
+  const double C = (double)x->ncell, N = (double)x->nnode, F = (double)x->nface;
+  const int m = x->o.nbmat, nc = x->o.nc;
+  const bool pp = x->o.pp == 1;
+  const int meth = x->o.methode;
+  double w = 0;
+  switch (k) {
+    case K_DT: w = C * 3 + N * 2; break;  // vol, c, p ; u
+    case K_NODE: w = N * 2 + C * 3 + N * 3; break;  // u_n ; p, q, M ; u_n+1, Mp
+    case K_CELL:
+      // u_n+1 ; f,mu,M_k,rho_k,e_k,p_k,q_k (7m) + rho, divu, c, vol ; writes UL, VL, XcL, YcL, divu, q_k (+rL, qL)
+      w = N * 2 + C * (7 * m + 4) + C * (nc + 4 + m + (meth == 1) + pp);
+      break;
+    case K_SWEEP_X:
+    case K_SWEEP_Y:
+      // U_in, VL, Xc (+qL, flags) ; u ; writes U_out, face fluxes (or PB)
+      w = C * (nc + 2 + pp + 2 * (pp && x->o.pp_mixte)) + N * 2 + C * nc + (meth == 2 ? C * m : F * m);
+      break;
+    case K_DUAL_X:
+    case K_DUAL_Y:
+      w = N * (2 + 1 + 3) + (meth == 2 ? C * m : F * m) + N * 3;
+      break;
+    case K_FINAL:
+      // U2 (+ void EOS inputs) ; writes rho, e, q, p, c, M, flags, f, mu, M_k, rho_k, e_k, p_k, c_k
+      w = C * nc + C * (5 + 1 + 1 + 7 * m);
+      break;
+  }
+  *bytes = w * 8.0;
+  return MHY_OK;
+}
+
+int mhy_enable_kernel_timing(mhy_ctx* x, int on) {
+  if (!x) return MHY_ERR_ARG;
+  (void)hipStreamSynchronize(x->stream);
+  collectTimes(x);
+  x->timing = (on != 0);
+  for (int k = 0; k < K_COUNT; ++k) x->ms[k] = 0, x->launches[k] = 0;
+  return MHY_OK;
+}
+
+int mhy_kernel_time(mhy_ctx* x, int k, double* avg_ms, int* launches) {
+  if (!x || k < 0 || k >= K_COUNT) return MHY_ERR_ARG;
+  (void)hipStreamSynchronize(x->stream);
+  collectTimes(x);
+  if (avg_ms) *avg_ms = x->launches[k] ? x->ms[k] / x->launches[k] : 0.0;
+  if (launches) *launches = x->launches[k];
+  return MHY_OK;
+}
+
+// The reference time loop with output (VnrRemap.cc:127-253): the last
+// iteration keeps its n+1 buffers unswapped, and dumps happen after
+// computeTime with the state of the step start.
+int mhy_simulate(mhy_ctx* x, const char* output_dir, int verbose) {
+  if (!x) return MHY_ERR_ARG;
+  if (x->status != MHY_OK) return x->status;
+  if (x->finished) return fail(x, MHY_ERR_FINISHED, "time loop already ended");
+  std::string od = output_dir ? output_dir : "none";
+  delete x->writer;
+  x->writer = new OutputWriter(od);
+  double lastDump = 0.0;
+  int nbCalls = 0;
+  x->iteration = 0;
+  bool cont = true;
+  std::vector<double> buf;
+  auto dump = [&](double tn) -> int {
+    nbCalls++;
+    if (x->writer->disabled() || !(tn >= lastDump + x->cfg.output_time || tn == 0.)) return MHY_OK;
+    int st = x->writer->write(x, nbCalls, tn);
+    lastDump = tn;
+    return st;
+  };
+  do {
+    int st = phaseDt(x);
+    if (st != MHY_OK) return st;
+    Scalars s;
+    if ((st = readScalars(x, s)) != MHY_OK) return st;
+    const double tn = s.t[x->cur];
+    if ((st = dump(tn)) != MHY_OK) return st;
+    if ((st = phaseLagrange(x)) != MHY_OK) return st;
+    if (x->o.mode != MODE_LAGRANGE) {
+      if ((st = phaseSweep(x, 1)) != MHY_OK) return st;
+      if ((st = phaseSweep(x, 2)) != MHY_OK) return st;
+      if ((st = phaseFinal(x)) != MHY_OK) return st;
+    }
+    x->iteration++;
+    if ((st = readScalars(x, s)) != MHY_OK) return st;
+    if (s.err) return fail(x, s.err, "kernel status " + std::to_string(s.err));
+    const double t1 = s.t[1 - x->cur];
+    cont = (x->iteration + 1 < x->cfg.max_time_iterations && t1 < x->cfg.final_time);
+    if (verbose)
+      std::printf("[%5d] t = %.8e DT = %.8e\n", x->iteration, tn, s.dt[1 - x->cur]);
+    if (cont) endIteration(x);
+  } while (cont);
+  {
+    Scalars s;
+    int st = readScalars(x, s);
+    if (st != MHY_OK) return st;
+    if ((st = dump(s.t[x->cur])) != MHY_OK) return st;
+  }
+  x->finished = true;
+  return MHY_OK;
+}
+
+}  // extern "C"
+
+// Field access for the output writer.
+namespace mhy {
+int ctxField(mhy_ctx* x, int field, std::vector<double>& out) {
+  long n = (field >= MHY_NODE_VELOCITY_X) ? x->nnode : x->ncell;
+  out.resize(n);
+  return mhy_get_field(x, field, out.data(), n);
+}
+int ctxNbmat(const mhy_ctx* x) { return x->o.nbmat; }
+int ctxProjection(const mhy_ctx* x) { return x->cfg.avec_projection; }
+void ctxMesh(const mhy_ctx* x, long& nx, long& ny) { nx = x->o.nx, ny = x->o.ny; }
+}  // namespace mhy

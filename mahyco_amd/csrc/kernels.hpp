@@ -1,0 +1,20 @@
+// Kernel entry points of the hot path (defined in lagrange.hip, remap.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace mhy {
+
+__global__ void k_dt_reduce(KP kp);
+__global__ void k_dt_hold(KP kp);
+__global__ void k_lagrange_node(KP kp);
+__global__ void k_lagrange_cell(KP kp);
+template <bool XD>
+__global__ void k_sweep(KP kp);
+template <bool XD>
+__global__ void k_dual(KP kp);
+__global__ void k_remap_final(KP kp);
+__global__ void k_node_mass(KP kp);
+__global__ void k_conservation(KP kp, double* out);
+
+}  // namespace mhy
