@@ -17,7 +17,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libmahyco_hip.so")
 ARCH = os.environ.get("MAHYCO_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["lagrange.hip", "remap.hip", "context.hip"]
+HIP_SOURCES = ["lagrange.hip", "sweep_cell.hip", "remap.hip", "context.hip"]
 HOST_SOURCES = ["host/donnees.cc", "host/init.cc", "host/output.cc"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -114,6 +114,7 @@ struct Fields {
   double *G[NMAT];   // face fluxes of the current sweep for the dual remap (A1/A2)
   double *DF[NMAT];  // PB dual flux per cell of the current sweep
   double *UD1[3];    // nodal vector after dual sweep 1 (M ux, M uy, M)
+  double *XL2;       // end-of-Lagrange node coordinate along the second sweep
   Scalars* sc;
   double* partials;  // per-block reduction partials (2 per block)
 };

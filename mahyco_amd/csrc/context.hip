@@ -177,18 +177,32 @@ int phaseLagrange(mhy_ctx* x) {
   return checkLaunch(x, "lagrange");
 }
 
+// sweep kernel for the direction, the material count and the PenteBorne option
+using KernelFn = void (*)(KP);
+KernelFn sweepKernel(bool xd, int m, bool pp) {
+  static const KernelFn kx[3][2] = {{k_sweep_cell<1, false, true>, k_sweep_cell<1, true, true>},
+                                    {k_sweep_cell<2, false, true>, k_sweep_cell<2, true, true>},
+                                    {k_sweep_cell<3, false, true>, k_sweep_cell<3, true, true>}};
+  static const KernelFn ky[3][2] = {{k_sweep_cell<1, false, false>, k_sweep_cell<1, true, false>},
+                                    {k_sweep_cell<2, false, false>, k_sweep_cell<2, true, false>},
+                                    {k_sweep_cell<3, false, false>, k_sweep_cell<3, true, false>}};
+  return xd ? kx[m - 1][pp ? 1 : 0] : ky[m - 1][pp ? 1 : 0];
+}
+
 int phaseSweep(mhy_ctx* x, int sweep) {
   KP kp = makeKP(x);
   kp.sweep = sweep;
   const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
   kp.xdir = xd ? 1 : 0;
   if (xd) {
+    const KernelFn k = sweepKernel(true, x->o.nbmat, x->o.pp == 1);
     timed(x, K_SWEEP_X,
-          [&] { hipLaunchKernelGGL(k_sweep<true>, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+          [&] { hipLaunchKernelGGL(k, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
     timed(x, K_DUAL_X, [&] { hipLaunchKernelGGL(k_dual<true>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
   } else {
+    const KernelFn k = sweepKernel(false, x->o.nbmat, x->o.pp == 1);
     timed(x, K_SWEEP_Y,
-          [&] { hipLaunchKernelGGL(k_sweep<false>, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+          [&] { hipLaunchKernelGGL(k, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
     timed(x, K_DUAL_Y,
           [&] { hipLaunchKernelGGL(k_dual<false>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
   }
@@ -371,6 +385,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     if (o.methode == 2)
       for (int k = 0; k < m; ++k) AD(f.DF[k], C);
     for (int d = 0; d < 3; ++d) AD(f.UD1[d], N);
+    AD(f.XL2, N);
   }
   if (e == hipSuccess) {
     void* p = nullptr;
@@ -619,7 +634,9 @@ int mhy_kernel_bytes(const mhy_ctx* x, int k, double* bytes) {
       break;
     case K_DUAL_X:
     case K_DUAL_Y:
-      w = N * (2 + 1 + 3) + (meth == 2 ? C * m : F * m) + N * 3;
+      // face fluxes (or PB cell fluxes) + sweep 1: u, Mp ; UD1, XL2 (7 N)
+      // or sweep 2: UD1, XL2 ; u (6 N)
+      w = N * 7 + (meth == 2 ? C * m : F * m);
       break;
     case K_FINAL:
       // U2 (+ void EOS inputs) ; writes rho, e, q, p, c, M, flags, f, mu, M_k, rho_k, e_k, p_k, c_k

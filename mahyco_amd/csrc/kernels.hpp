@@ -1,4 +1,5 @@
-// Kernel entry points of the hot path (defined in lagrange.hip, remap.hip).
+// Kernel entry points of the hot path (defined in lagrange.hip,
+// sweep_cell.hip, remap.hip).
 #pragma once
 
 #include "common.hpp"
@@ -9,8 +10,8 @@ __global__ void k_dt_reduce(KP kp);
 __global__ void k_dt_hold(KP kp);
 __global__ void k_lagrange_node(KP kp);
 __global__ void k_lagrange_cell(KP kp);
-template <bool XD>
-__global__ void k_sweep(KP kp);
+template <int M, bool PP, bool XD>
+__global__ void k_sweep_cell(KP kp);
 template <bool XD>
 __global__ void k_dual(KP kp);
 __global__ void k_remap_final(KP kp);

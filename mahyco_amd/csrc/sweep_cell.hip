@@ -1,0 +1,255 @@
+// Remap sweeps, one thread per cell (Remap::computeGradPhiFace{1,2},
+// computeGradPhi{1,2}, computeUpwindFaceQuantitiesForProjection{1,2},
+// computeUremap{1,2}; remap/PhaseRemap1.cc, remap/PhaseRemap2.cc).
+//
+// Each thread reads the 5 cells a-2 .. a+2 along the sweep (neighbour reads
+// hit L1/L2) and forms what its two faces need: the limited slope of the
+// upwind cell of each face, or the PenteBorne fluxes Av(a-1), Ar(a), Av(a),
+// Ar(a+1).  The material count is a template parameter, so the 5 x (3m+1)
+// stencil values live in registers.  Operation order follows the reference,
+// so FP64 results are bitwise those of the CPU path.  The face fluxes are
+// also written per face for the nodal (dual) momentum remap.
+#include "kernels.hpp"
+#include "sweep_math.hpp"
+
+namespace mhy {
+
+using namespace sw;
+
+namespace {
+
+// end-of-Lagrange node position (updatePosition, PhaseLagrange.cc:203-208)
+__device__ __forceinline__ D2 lagNodeC(const KP& kp, double dt1, long j, long i) {
+  const long p = j * (kp.o.nx + 1) + i;
+  const D2 e = eulerNode(kp.o, j, i);
+  return {e.x + kp.f.ux[kp.nxt][p] * dt1, e.y + kp.f.uy[kp.nxt][p] * dt1};
+}
+
+// width across the sweep faces (HvLagrange, PhaseRemap1.cc:57-71, :104-119)
+template <bool XD>
+__device__ __forceinline__ double widthC(const KP& kp, double dt1, long j, long i) {
+  double h = 0.;
+  if (XD) {  // bottom then top face
+    h += 0.5 * d2norm(d2sub(lagNodeC(kp, dt1, j, i + 1), lagNodeC(kp, dt1, j, i)));
+    h += 0.5 * d2norm(d2sub(lagNodeC(kp, dt1, j + 1, i + 1), lagNodeC(kp, dt1, j + 1, i)));
+  } else {  // right then left face
+    h += 0.5 * d2norm(d2sub(lagNodeC(kp, dt1, j + 1, i + 1), lagNodeC(kp, dt1, j, i + 1)));
+    h += 0.5 * d2norm(d2sub(lagNodeC(kp, dt1, j + 1, i), lagNodeC(kp, dt1, j, i)));
+  }
+  return h;
+}
+
+template <int NC>
+struct Slot {
+  double phi[NC];
+  double xc, H;
+  int fl;
+};
+
+template <int NC>
+__device__ __forceinline__ void copySlot(Slot<NC>& d, const Slot<NC>& s) {
+#pragma unroll
+  for (int q = 0; q < NC; ++q) d.phi[q] = s.phi[q];
+  d.xc = s.xc, d.H = s.H, d.fl = s.fl;
+}
+
+// per-lane choice between two slots, by value (keeps everything in registers)
+template <int NC>
+__device__ __forceinline__ void selectSlot(Slot<NC>& d, bool first, const Slot<NC>& s0, const Slot<NC>& s1) {
+#pragma unroll
+  for (int q = 0; q < NC; ++q) d.phi[q] = first ? s0.phi[q] : s1.phi[q];
+  d.xc = first ? s0.xc : s1.xc;
+  d.H = first ? s0.H : s1.H;
+  d.fl = first ? s0.fl : s1.fl;
+}
+
+__device__ __forceinline__ bool pureNbh(int fm, int f0, int fp) { return ((f0 >> 8) == 0) && f0 == fp && f0 == fm; }
+
+}  // namespace
+
+template <int M, bool PP, bool XD>
+__global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
+  constexpr int NC = 3 * M + 1;
+  const Opts& o = kp.o;
+  const Fields& f = kp.f;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= kp.ncell) return;
+  const long nx = o.nx, ny = o.ny;
+  const long j = c / nx, i = c - j * nx;
+  const long a = XD ? i : j, na = XD ? nx : ny, stride = XD ? 1 : nx;
+  double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
+  double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
+  const double dtn = f.sc->dt[kp.cur], dt1 = f.sc->dt[kp.nxt];
+  const bool needH = PP || o.limiter >= MHY_LIM_MINMODG;
+  const bool mixteOpt = PP && o.pp_mixte == 1;
+  const bool pb = PP && o.methode == 2;
+
+  // stencil slots 0..4 = cells a-2 .. a+2
+  Slot<NC> s[5];
+  bool in[5];
+#pragma unroll
+  for (int d = 0; d < 5; ++d) {
+    const long b = a + d - 2;
+    in[d] = (b >= 0 && b < na);
+    if (!in[d]) continue;
+    const long cd = c + (d - 2) * stride;
+    double u[NC];
+    const int nload = PP ? 3 * M : NC;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cd] : 0.0;
+    cellPhi<M, PP>(u, PP ? 1.0 : f.VL[cd], PP ? f.qL[cd] : 0.0, s[d].phi);
+    s[d].xc = XD ? f.XcL[cd] : f.YcL[cd];
+    s[d].H = needH ? widthC<XD>(kp, dt1, XD ? j : b, XD ? b : i) : 0.0;
+    s[d].fl = 0;
+    if (mixteOpt) s[d].fl = (kp.sweep == 1) ? packFlags(f.mixte[cd], f.pure[cd]) : packedFlags<M>(s[d].phi);
+  }
+  // a missing neighbour takes the values of the cell it neighbours
+  // (PhaseRemap1.cc:157-158)
+  if (!in[1]) copySlot(s[1], s[2]);
+  if (!in[3]) copySlot(s[3], s[2]);
+  if (!in[0]) copySlot(s[0], s[1]);
+  if (!in[4]) copySlot(s[4], s[3]);
+
+  // neighbourhood of slot d (1..3)
+  auto nbr = [&](const Slot<NC>& sm, const Slot<NC>& s0, const Slot<NC>& sp, bool inM, bool inP) {
+    Nbr n{sm.phi, s0.phi, sp.phi, sm.xc, s0.xc, sp.xc, inM, inP, s0.H, sm.H, sp.H, false};
+    if (mixteOpt) n.vp = pureNbh(sm.fl, s0.fl, sp.fl);
+    return n;
+  };
+
+  // face normal velocity of face b, between along-cells b-1 and b
+  // (PrepareRemap.cc:50-62)
+  auto fnv = [&](long b) {
+    long n1, n2;
+    if (XD) n1 = j * (nx + 1) + b, n2 = (j + 1) * (nx + 1) + b;
+    else n1 = b * (nx + 1) + i, n2 = b * (nx + 1) + i + 1;
+    const double* u = XD ? f.ux[kp.nxt] : f.uy[kp.nxt];
+    return 0.0 + ((0.0 + u[n1]) + u[n2]) * 0.5;
+  };
+  // Euler length of face b (Init.cc:528-533)
+  auto faceLen = [&](long b) {
+    D2 e1, e2;
+    if (XD) e1 = eulerNode(o, j, b), e2 = eulerNode(o, j + 1, b);
+    else e1 = eulerNode(o, b, i), e2 = eulerNode(o, b, i + 1);
+    return d2norm(d2sub(e2, e1));
+  };
+
+  const bool hasM = (a >= 1), hasP = (a + 1 < na);
+  const double Lm = faceLen(a), Lp = faceLen(a + 1);
+  const double vm = fnv(a), vp = fnv(a + 1);
+  double phiM[NC], phiP[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) phiM[q] = phiP[q] = 0.0;
+  if (!PP) {
+    // upwind value (UtilesRemap-Impl.h:491-501) of a face between slots
+    // lo and lo+1; back/front cells: x: left/right, y: above/below
+    auto faceValue = [&](long b, const Slot<NC>& sl0, const Slot<NC>& sl1, const Slot<NC>& sl2, const Slot<NC>& sl3,
+                         bool in0, bool in1, bool in2, bool in3, double v, double* out) {
+      // sl1 = lower along-cell, sl2 = upper along-cell
+      const double dxl = XD ? sl2.xc - sl1.xc : sl1.xc - sl2.xc;  // front - back
+      const bool upLow = XD ? (v * dxl > 0.0) : !(v * dxl > 0.0);
+      Slot<NC> um, u0, up;
+      selectSlot(um, upLow, sl0, sl1);
+      selectSlot(u0, upLow, sl1, sl2);
+      selectSlot(up, upLow, sl2, sl3);
+      const bool inM = upLow ? in0 : in1, inP = upLow ? in2 : in3;
+      const Nbr n = nbr(um, u0, up, inM, inP);
+      const double xf = XD ? 0.5 * (o.dx * (double)b + o.dx * (double)b) : 0.5 * (o.dy * (double)b + o.dy * (double)b);
+      const double sx = 0.0 + (xf - u0.xc);
+#pragma unroll
+      for (int q = 0; q < NC; ++q) out[q] = u0.phi[q] + gradQ<XD>(o, n, q) * sx;
+    };
+    if (hasM) faceValue(a, s[0], s[1], s[2], s[3], in[0], in[1], in[2], in[3], vm, phiM);
+    if (hasP) faceValue(a + 1, s[1], s[2], s[3], s[4], in[1], in[2], in[3], in[4], vp, phiP);
+  } else if (o.order > 1) {
+    // PenteBorne: phi_f = Av(minus cell) - Ar(plus cell) (PhaseRemap1.cc:361-364)
+    double Ar0[NC], Av0[NC], Dual[M];
+    ppFluxes<M, XD>(o, nbr(s[1], s[2], s[3], in[1], in[3]), vm, vp, dtn, pb, Ar0, Av0, Dual);
+    if (pb) {
+      const double w = (Lm + Lp) * 0.5;
+#pragma unroll
+      for (int k = 0; k < M; ++k) f.DF[k][c] = Dual[k] * w;
+    }
+    if (hasM) {
+      double ArM[NC], AvM[NC], DM[M];
+      ppFluxes<M, XD, false, true>(o, nbr(s[0], s[1], s[2], in[0], in[2]), fnv(a - 1), vm, dtn, false, ArM, AvM, DM);
+#pragma unroll
+      for (int q = 0; q < NC; ++q) phiM[q] = AvM[q] - Ar0[q];
+    }
+    if (hasP) {
+      double ArP[NC], AvP[NC], DP[M];
+      ppFluxes<M, XD, true, false>(o, nbr(s[2], s[3], s[4], in[2], in[4]), vp, fnv(a + 2), dtn, false, ArP, AvP, DP);
+#pragma unroll
+      for (int q = 0; q < NC; ++q) phiP[q] = Av0[q] - ArP[q];
+    }
+  } else if (pb) {
+    // the PenteBorne fluxes are only formed for order > 1
+#pragma unroll
+    for (int k = 0; k < M; ++k) f.DF[k][c] = 0.0;
+  }
+
+  // computeUremap (PhaseRemap1.cc:468-503): a cell receives -G through its
+  // minus face and +G through its plus face
+  const double coefM = PP ? Lm : vm * Lm * dtn;
+  const double coefP = PP ? Lp : vp * Lp * dtn;
+  double Gm[NC], red[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    Gm[q] = coefM * phiM[q];
+    const double Gp = coefP * phiP[q];
+    red[q] = 0.0;
+    if (hasM) red[q] = red[q] + (-Gm[q]);
+    if (hasP) red[q] = red[q] + Gp;
+  }
+  if (o.flux_any) {
+    // remap/BoundaryFluxes.cc:14-120: flux through one face of the cell with
+    // the face value of the opposite face; the first matching rule applies
+    int rule = 0;  // 1: through the minus face, 2: through the plus face
+    if (!XD && o.flux_bottom == 1 && c < nx) rule = 1;
+    else if (!XD && o.flux_top == 1 && c <= nx * (ny - 1) && c < nx * ny) rule = 2;
+    else if (XD && o.flux_left == 1 && i == 0) rule = 1;
+    else if (XD && o.flux_right == 1 && i == nx - 1) rule = 2;
+    if (rule != 0) {
+      const bool viaMinus = (rule == 1);
+      const bool oppOk = viaMinus ? hasP : hasM;
+      const double vb = viaMinus ? vm : vp;
+      const double Lb = viaMinus ? Lm : Lp;
+      const double sgn = viaMinus ? -1.0 : 1.0;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        const double ph = oppOk ? (viaMinus ? phiP[q] : phiM[q]) : 0.0;
+        const double Fb = PP ? (sgn * Lb) * ph : (((sgn * vb) * Lb) * dtn) * ph;
+        red[q] = red[q] + Fb;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) Uout[q][c] = Uin[q][c] - red[q];
+
+  // face fluxes for the dual remap (A1: mass, A2: volume): the minus face of
+  // each cell, plus the last face of the line
+  if (o.methode == 0 || o.methode == 1) {
+    const int q0 = (o.methode == 0) ? M : 0;
+    const long fm = XD ? j * (nx + 1) + a : a * nx + i;
+#pragma unroll
+    for (int k = 0; k < M; ++k) f.G[k][fm] = hasM ? Gm[q0 + k] : 0.0;
+    if (!hasP) {
+      const long fp = XD ? j * (nx + 1) + a + 1 : (a + 1) * nx + i;
+#pragma unroll
+      for (int k = 0; k < M; ++k) f.G[k][fp] = 0.0;
+    }
+  }
+}
+
+#define MHY_SWEEP_CELL_INST(M, PP)                       \
+  template __global__ void k_sweep_cell<M, PP, true>(KP); \
+  template __global__ void k_sweep_cell<M, PP, false>(KP);
+MHY_SWEEP_CELL_INST(1, false)
+MHY_SWEEP_CELL_INST(1, true)
+MHY_SWEEP_CELL_INST(2, false)
+MHY_SWEEP_CELL_INST(2, true)
+MHY_SWEEP_CELL_INST(3, false)
+MHY_SWEEP_CELL_INST(3, true)
+#undef MHY_SWEEP_CELL_INST
+
+}  // namespace mhy
