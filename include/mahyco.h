@@ -201,11 +201,29 @@ int mhy_run(mhy_ctx* ctx, int nsteps);
  * output_dir (NULL or "none": no output) -- the drop-in for Vnr::simulate. */
 int mhy_simulate(mhy_ctx* ctx, const char* output_dir, int verbose);
 
+/* Output of Vnr::dumpVariables (vnr_remap/VnrRemap.cc:255-299) through the
+ * reference's PvdFileWriter2D format (cppnabla/mesh/PvdFileWriter2D.cc:26-114):
+ * mhy_output_open selects the directory ("none" disables; the directory is
+ * created if missing), mhy_output_write writes <dir>/VnrRemap.<index>.vtp for
+ * the current state at time t and rewrites <dir>/VnrRemap.pvd. */
+int mhy_output_open(mhy_ctx* ctx, const char* output_dir);
+int mhy_output_write(mhy_ctx* ctx, int index, double t);
+
 /* Phase-level entry points, 1:1 with the reference call sequence. */
 int mhy_compute_dt(mhy_ctx* ctx);     /* computeCornerNormal+computeDeltaT+computeTime */
 int mhy_lagrange(mhy_ctx* ctx);       /* updateVelocity .. computePressionMoyenne */
 int mhy_prepare_remap(mhy_ctx* ctx);  /* PrepareRemap.cc                          */
 int mhy_remap_sweep(mhy_ctx* ctx, int sweep); /* sweep 1 or 2: cells + dual      */
+/* One stage of a sweep.  MHY_STAGE_CELLS replaces Remap::computeGradPhiFace{1,2},
+ * computeGradPhi{1,2}, computeUpwindFaceQuantitiesForProjection{1,2} and
+ * computeUremap{1,2} (remap/Remap.h:161-168, remap/PhaseRemap1.cc,
+ * remap/PhaseRemap2.cc), fused in one kernel; MHY_STAGE_NODES replaces
+ * Remap::computeDualUremap{1,2} (remap/Remap.h:169-170,
+ * remap/DualPhaseRemap1.cc, remap/DualPhaseRemap2.cc).  Stage 2 of a sweep
+ * reads the face fluxes of stage 1 of the same sweep. */
+#define MHY_STAGE_CELLS 1
+#define MHY_STAGE_NODES 2
+int mhy_remap_stage(mhy_ctx* ctx, int sweep, int stage);
 int mhy_remap_final(mhy_ctx* ctx);    /* remapVariables + node mass + EOS         */
 int mhy_end_iteration(mhy_ctx* ctx);  /* loop test + swaps (VnrRemap.cc:193-222)  */
 

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libmahyco_hip.so")
 # every entry point declared in include/mahyco.h
 EXPORTS = [
     "mhy_read_donnees", "mhy_config_default", "mhy_create", "mhy_destroy", "mhy_last_error", "mhy_step",
-    "mhy_run", "mhy_simulate", "mhy_compute_dt", "mhy_lagrange", "mhy_prepare_remap", "mhy_remap_sweep",
+    "mhy_run", "mhy_simulate", "mhy_output_open", "mhy_output_write", "mhy_compute_dt", "mhy_lagrange",
+    "mhy_prepare_remap", "mhy_remap_sweep", "mhy_remap_stage",
     "mhy_remap_final", "mhy_end_iteration", "mhy_synchronize", "mhy_get_time_state", "mhy_get_mesh_size",
     "mhy_get_field", "mhy_set_field", "mhy_conservation", "mhy_kernel_count", "mhy_kernel_name",
     "mhy_kernel_bytes", "mhy_enable_kernel_timing", "mhy_kernel_time",
@@ -52,6 +53,9 @@ def library():
                      "mhy_end_iteration", "mhy_synchronize"):
             getattr(L, name).argtypes = [P]
         L.mhy_remap_sweep.argtypes = [P, ctypes.c_int]
+        L.mhy_remap_stage.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.mhy_output_open.argtypes = [P, ctypes.c_char_p]
+        L.mhy_output_write.argtypes = [P, ctypes.c_int, ctypes.c_double]
         L.mhy_get_time_state.argtypes = [P, ctypes.POINTER(MhyTimeState)]
         L.mhy_get_mesh_size.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.mhy_get_field.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_longlong]

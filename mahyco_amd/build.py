@@ -15,6 +15,9 @@ SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libmahyco_hip.so")
+BIN_DIR = os.path.join(HERE, "bin")
+EXE = os.path.join(BIN_DIR, "mahyco")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ARCH = os.environ.get("MAHYCO_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["lagrange.hip", "sweep_cell.hip", "remap.hip", "context.hip"]
@@ -64,7 +67,25 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+def build_driver():
+    """The mahyco executable (C++ drivers of include/mahyco.hpp over the C
+    ABI), linked against the in-tree library."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        raise RuntimeError("no host C++ compiler for the driver executable")
+    tmp = EXE + ".tmp"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, os.path.join(SRC, "drivers", "main.cc"), "-o", tmp,
+           "-L", LIB_DIR, "-lmahyco_hip", "-Wl,-rpath,$ORIGIN/../lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, EXE)
+    return EXE
 
 
 if __name__ == "__main__":

@@ -189,24 +189,29 @@ KernelFn sweepKernel(bool xd, int m, bool pp) {
   return xd ? kx[m - 1][pp ? 1 : 0] : ky[m - 1][pp ? 1 : 0];
 }
 
-int phaseSweep(mhy_ctx* x, int sweep) {
+// One remap sweep: stage 1 = cell fluxes and update (computeGradPhiFace ..
+// computeUremap), stage 2 = nodal momentum remap (computeDualUremap).
+int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   KP kp = makeKP(x);
   kp.sweep = sweep;
   const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
   kp.xdir = xd ? 1 : 0;
-  if (xd) {
-    const KernelFn k = sweepKernel(true, x->o.nbmat, x->o.pp == 1);
-    timed(x, K_SWEEP_X,
+  if (stage == MHY_STAGE_CELLS) {
+    const KernelFn k = sweepKernel(xd, x->o.nbmat, x->o.pp == 1);
+    timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
           [&] { hipLaunchKernelGGL(k, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+  } else if (xd) {
     timed(x, K_DUAL_X, [&] { hipLaunchKernelGGL(k_dual<true>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
   } else {
-    const KernelFn k = sweepKernel(false, x->o.nbmat, x->o.pp == 1);
-    timed(x, K_SWEEP_Y,
-          [&] { hipLaunchKernelGGL(k, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
     timed(x, K_DUAL_Y,
           [&] { hipLaunchKernelGGL(k_dual<false>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
   }
   return checkLaunch(x, "sweep");
+}
+
+int phaseSweep(mhy_ctx* x, int sweep) {
+  int st = phaseSweepStage(x, sweep, MHY_STAGE_CELLS);
+  return st != MHY_OK ? st : phaseSweepStage(x, sweep, MHY_STAGE_NODES);
 }
 
 int phaseFinal(mhy_ctx* x) {
@@ -495,6 +500,23 @@ int mhy_remap_sweep(mhy_ctx* x, int sweep) {
   if (!x || (sweep != 1 && sweep != 2)) return MHY_ERR_ARG;
   if (x->o.mode == MODE_LAGRANGE) return MHY_OK;
   return phaseSweep(x, sweep);
+}
+int mhy_remap_stage(mhy_ctx* x, int sweep, int stage) {
+  if (!x || (sweep != 1 && sweep != 2) || (stage != MHY_STAGE_CELLS && stage != MHY_STAGE_NODES))
+    return MHY_ERR_ARG;
+  if (x->o.mode == MODE_LAGRANGE) return MHY_OK;
+  return phaseSweepStage(x, sweep, stage);
+}
+int mhy_output_open(mhy_ctx* x, const char* output_dir) {
+  if (!x) return MHY_ERR_ARG;
+  delete x->writer;
+  x->writer = new OutputWriter(output_dir ? output_dir : "none");
+  return MHY_OK;
+}
+int mhy_output_write(mhy_ctx* x, int index, double t) {
+  if (!x || !x->writer) return MHY_ERR_ARG;
+  if (x->status != MHY_OK) return x->status;
+  return x->writer->write(x, index, t);
 }
 int mhy_remap_final(mhy_ctx* x) {
   if (!x) return MHY_ERR_ARG;
