@@ -2125,6 +2125,30 @@ ref_ctx* ref_create(const mhy_config* cfg, int* status) {
 
 void ref_destroy(ref_ctx* r) { delete r; }
 
+// Single-formula probes for tests/test_oracle_math.py (the oracle's own
+// functions, evaluated on given inputs).
+double ref_probe_gradient(ref_ctx* r, int lim, double gp, double gm, double phi, double phip, double phim, double h0,
+                          double hp, double hm) {
+  VE vgp = ezero(), vgm = ezero(), vphi = ezero(), vphip = ezero(), vphim = ezero();
+  vgp[0] = gp, vgm[0] = gm, vphi[0] = phi, vphip[0] = phip, vphim[0] = phim;
+  return r->computeAndLimitGradPhi(lim, vgp, vgm, vphi, vphip, vphim, h0, hp, hm)[0];
+}
+
+double ref_probe_int2y(double X, double x0, double y0, double x1, double y1) {
+  return ref_ctx::INT2Y(X, x0, y0, x1, y1);
+}
+
+double ref_probe_flux_pp(ref_ctx* r, int lim, double grad, double phi, double phip, double phim, double h0, double hp,
+                         double hm, double fnv, double dt, int type) {
+  VE g = ezero(), v = ezero(), vp = ezero(), vm = ezero(), flux = ezero(), dual = ezero();
+  g[0] = grad, v[0] = phi, vp[0] = phip, vm[0] = phim;
+  const int keep = r->o.limiter;
+  r->o.limiter = lim;
+  r->computeFluxPP(false, g, v, vp, vm, h0, hp, hm, fnv, dt, type, r->o.threshold, 0, 0.0, 0, &flux, &dual);
+  r->o.limiter = keep;
+  return flux[0];
+}
+
 int ref_step(ref_ctx* r) {
   if (r->status != MHY_OK) return r->status;
   bool cont = true;

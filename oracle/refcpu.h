@@ -31,6 +31,16 @@ int ref_step(ref_ctx* r);
  * disabled).  Returns MHY_OK or an error status. */
 int ref_simulate(ref_ctx* r, const char* output_dir, int max_steps);
 
+/* Single-formula probes (tests only): the limited gradient of one component
+ * (remap/UtilesRemap-Impl.h:99-126), INT2Y (remap/UtilesRemap.cc:177-188),
+ * and the PenteBorne swept flux of one component (UtilesRemap-Impl.h:142-316;
+ * type 0: back face, 1: front face) under limiter lim. */
+double ref_probe_gradient(ref_ctx* r, int lim, double gp, double gm, double phi, double phip, double phim, double h0,
+                          double hp, double hm);
+double ref_probe_int2y(double X, double x0, double y0, double x1, double y1);
+double ref_probe_flux_pp(ref_ctx* r, int lim, double grad, double phi, double phip, double phim, double h0, double hp,
+                         double hm, double fnv, double dt, int type);
+
 int ref_get_field(ref_ctx* r, int field, double* out, long long n);
 int ref_get_time_state(ref_ctx* r, mhy_time_state* ts);
 int ref_set_threads(int nthreads);

@@ -45,6 +45,13 @@ def lib():
                                     ctypes.c_longlong]
         L.ref_get_time_state.argtypes = [ctypes.c_void_p, ctypes.POINTER(MhyTimeState)]
         L.ref_set_threads.argtypes = [ctypes.c_int]
+        d = ctypes.c_double
+        L.ref_probe_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int] + [d] * 8
+        L.ref_probe_gradient.restype = d
+        L.ref_probe_int2y.argtypes = [d] * 5
+        L.ref_probe_int2y.restype = d
+        L.ref_probe_flux_pp.argtypes = [ctypes.c_void_p, ctypes.c_int] + [d] * 9 + [ctypes.c_int]
+        L.ref_probe_flux_pp.restype = d
         _lib = L
     return _lib
 
@@ -55,6 +62,11 @@ def read_donnees(path):
     if st != MHY_OK:
         raise RuntimeError(f"cannot read {path}: {st}")
     return cfg
+
+
+def probe_int2y(X, x0, y0, x1, y1):
+    """INT2Y (remap/UtilesRemap.cc:177-188)."""
+    return lib().ref_probe_int2y(X, x0, y0, x1, y1)
 
 
 def set_threads(n):
@@ -101,6 +113,14 @@ class Oracle:
         if name in MAT_FIELDS:
             return mat_field(MAT_FIELDS[name], k), self.nx * self.ny
         return NODE_FIELDS[name], (self.nx + 1) * (self.ny + 1)
+
+    def probe_gradient(self, lim, gp, gm, phi, phip, phim, h0, hp, hm):
+        """Limited gradient of one component (remap/UtilesRemap-Impl.h:99-126)."""
+        return lib().ref_probe_gradient(self.h, lim, gp, gm, phi, phip, phim, h0, hp, hm)
+
+    def probe_flux_pp(self, lim, grad, phi, phip, phim, h0, hp, hm, fnv, dt, face):
+        """PenteBorne swept flux of one component; face 0 = back, 1 = front."""
+        return lib().ref_probe_flux_pp(self.h, lim, grad, phi, phip, phim, h0, hp, hm, fnv, dt, face)
 
     def get(self, name, k=0):
         fid, n = self.field_id(name, k)
