@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -40,6 +41,9 @@ struct mhy_ctx {
   bool x_then_y_n = true;
   int iteration = 0;
   bool finished = false;
+  // sweep kernels: wave-shuffle (default) or per-cell stencil
+  // (environment MAHYCO_SWEEP_KERNEL=cell selects the latter)
+  bool sweepWave = true;
   int status = MHY_OK;
   long ncell = 0, nnode = 0, nface = 0;
   std::string err;
@@ -177,16 +181,25 @@ int phaseLagrange(mhy_ctx* x) {
   return checkLaunch(x, "lagrange");
 }
 
-// sweep kernel for the direction, the material count and the PenteBorne option
+// sweep kernels for the direction, the material count and the PenteBorne
+// option: the wave-shuffle kernels (default) or the per-cell stencil kernels
 using KernelFn = void (*)(KP);
-KernelFn sweepKernel(bool xd, int m, bool pp) {
-  static const KernelFn kx[3][2] = {{k_sweep_cell<1, false, true>, k_sweep_cell<1, true, true>},
-                                    {k_sweep_cell<2, false, true>, k_sweep_cell<2, true, true>},
-                                    {k_sweep_cell<3, false, true>, k_sweep_cell<3, true, true>}};
-  static const KernelFn ky[3][2] = {{k_sweep_cell<1, false, false>, k_sweep_cell<1, true, false>},
-                                    {k_sweep_cell<2, false, false>, k_sweep_cell<2, true, false>},
-                                    {k_sweep_cell<3, false, false>, k_sweep_cell<3, true, false>}};
-  return xd ? kx[m - 1][pp ? 1 : 0] : ky[m - 1][pp ? 1 : 0];
+KernelFn sweepKernel(bool wave, bool xd, int m, bool pp) {
+  static const KernelFn kw[2][3][2] = {
+      {{k_sweep_wave<1, false, false>, k_sweep_wave<1, true, false>},
+       {k_sweep_wave<2, false, false>, k_sweep_wave<2, true, false>},
+       {k_sweep_wave<3, false, false>, k_sweep_wave<3, true, false>}},
+      {{k_sweep_wave<1, false, true>, k_sweep_wave<1, true, true>},
+       {k_sweep_wave<2, false, true>, k_sweep_wave<2, true, true>},
+       {k_sweep_wave<3, false, true>, k_sweep_wave<3, true, true>}}};
+  static const KernelFn kc[2][3][2] = {
+      {{k_sweep_cell<1, false, false>, k_sweep_cell<1, true, false>},
+       {k_sweep_cell<2, false, false>, k_sweep_cell<2, true, false>},
+       {k_sweep_cell<3, false, false>, k_sweep_cell<3, true, false>}},
+      {{k_sweep_cell<1, false, true>, k_sweep_cell<1, true, true>},
+       {k_sweep_cell<2, false, true>, k_sweep_cell<2, true, true>},
+       {k_sweep_cell<3, false, true>, k_sweep_cell<3, true, true>}}};
+  return (wave ? kw : kc)[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
 
 // One remap sweep: stage 1 = cell fluxes and update (computeGradPhiFace ..
@@ -197,9 +210,9 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
   kp.xdir = xd ? 1 : 0;
   if (stage == MHY_STAGE_CELLS) {
-    const KernelFn k = sweepKernel(xd, x->o.nbmat, x->o.pp == 1);
-    timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
-          [&] { hipLaunchKernelGGL(k, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+    const KernelFn k = sweepKernel(x->sweepWave, xd, x->o.nbmat, x->o.pp == 1);
+    const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny) : gridFor(x->ncell);
+    timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, x->stream, kp); });
   } else if (xd) {
     timed(x, K_DUAL_X, [&] { hipLaunchKernelGGL(k_dual<true>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
   } else {
@@ -306,6 +319,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   *out = nullptr;
   mhy_ctx* x = new mhy_ctx();
   x->cfg = *cfg;
+  if (const char* sk = std::getenv("MAHYCO_SWEEP_KERNEL")) x->sweepWave = (std::string(sk) != "cell");
   const mhy_config& c = *cfg;
   auto bail = [&](int code) {
     // keep the context so the caller can read the message, unless it is

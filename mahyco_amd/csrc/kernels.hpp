@@ -1,5 +1,5 @@
 // Kernel entry points of the hot path (defined in lagrange.hip,
-// sweep_cell.hip, remap.hip).
+// sweep_wave.hip, sweep_cell.hip, remap.hip).
 #pragma once
 
 #include "common.hpp"
@@ -12,6 +12,9 @@ __global__ void k_lagrange_node(KP kp);
 __global__ void k_lagrange_cell(KP kp);
 template <int M, bool PP, bool XD>
 __global__ void k_sweep_cell(KP kp);
+template <int M, bool PP, bool XD>
+__global__ void k_sweep_wave(KP kp);
+dim3 sweepWaveGrid(bool xd, long nx, long ny);
 template <bool XD>
 __global__ void k_dual(KP kp);
 __global__ void k_remap_final(KP kp);

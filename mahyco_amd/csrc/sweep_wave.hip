@@ -1,0 +1,277 @@
+// Remap sweeps (Remap::computeGradPhiFace{1,2}, computeGradPhi{1,2},
+// computeUpwindFaceQuantitiesForProjection{1,2}, computeUremap{1,2};
+// remap/PhaseRemap1.cc, remap/PhaseRemap2.cc), one lane per cell, with the
+// neighbours' values exchanged inside the 64-wide wavefront.
+//
+// A wave covers a strip of cells along the sweep direction plus one halo
+// cell at each end: along x, 64 consecutive cells of a row (62 updated);
+// along y, 4 columns x 16 rows (14 rows updated), so that every row access
+// still reads 4 consecutive cells and a 256-thread block reads 16.  Each lane
+// loads only its own cell, forms its own limited slope (or its own PenteBorne
+// fluxes Ar and Av) once, and takes the neighbour values it needs with wave
+// shuffles: no LDS is allocated.  Operation order follows the reference, so
+// FP64 results are bitwise those of the CPU path.
+#include "kernels.hpp"
+#include "sweep_math.hpp"
+
+namespace mhy {
+
+using namespace sw;
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kXOut = kWave - 2;   // cells updated per wave along x
+constexpr int kYCols = 4;          // columns per wave along y
+constexpr int kYRows = kWave / kYCols;
+constexpr int kYOut = kYRows - 2;  // rows updated per wave along y
+
+// end-of-Lagrange node position (updatePosition, PhaseLagrange.cc:203-208)
+__device__ __forceinline__ D2 lagNodeW(const KP& kp, double dt1, long j, long i) {
+  const long p = j * (kp.o.nx + 1) + i;
+  const D2 e = eulerNode(kp.o, j, i);
+  return {e.x + kp.f.ux[kp.nxt][p] * dt1, e.y + kp.f.uy[kp.nxt][p] * dt1};
+}
+
+// width across the sweep faces (HvLagrange, PhaseRemap1.cc:57-71, :104-119)
+template <bool XD>
+__device__ __forceinline__ double widthW(const KP& kp, double dt1, long j, long i) {
+  double h = 0.;
+  if (XD) {  // bottom then top face
+    h += 0.5 * d2norm(d2sub(lagNodeW(kp, dt1, j, i + 1), lagNodeW(kp, dt1, j, i)));
+    h += 0.5 * d2norm(d2sub(lagNodeW(kp, dt1, j + 1, i + 1), lagNodeW(kp, dt1, j + 1, i)));
+  } else {  // right then left face
+    h += 0.5 * d2norm(d2sub(lagNodeW(kp, dt1, j + 1, i + 1), lagNodeW(kp, dt1, j, i + 1)));
+    h += 0.5 * d2norm(d2sub(lagNodeW(kp, dt1, j + 1, i), lagNodeW(kp, dt1, j, i)));
+  }
+  return h;
+}
+
+__device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, kWave); }
+__device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, kWave); }
+
+}  // namespace
+
+template <int M, bool PP, bool XD>
+__global__ __launch_bounds__(256) void k_sweep_wave(KP kp) {
+  constexpr int NC = 3 * M + 1;
+  constexpr int S = XD ? 1 : kYCols;  // lane distance of the along-sweep neighbour
+  const Opts& o = kp.o;
+  const Fields& f = kp.f;
+  const long nx = o.nx, ny = o.ny;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+
+  // cell of this lane: along index a (with na cells), across index t
+  long a, t, na, nt;
+  int slot;  // position along the sweep inside the wave strip (0 and last: halo)
+  if (XD) {
+    const long seg = (long)blockIdx.x * 4 + wave;
+    slot = lane;
+    a = seg * kXOut - 1 + lane;
+    t = blockIdx.y;
+    na = nx, nt = ny;
+  } else {
+    slot = lane / kYCols;
+    a = (long)blockIdx.y * kYOut - 1 + slot;
+    t = (long)blockIdx.x * (4 * kYCols) + wave * kYCols + (lane % kYCols);
+    na = ny, nt = nx;
+  }
+  const bool inA = (a >= 0 && a < na), inT = (t < nt);
+  const bool in = inA && inT;
+  const long j = XD ? t : a, i = XD ? a : t;
+  const long c = j * nx + i;
+  const int lastSlot = XD ? kWave - 1 : kYRows - 1;
+  const bool output = in && slot >= 1 && slot < lastSlot;
+
+  double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
+  double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
+  const double dtn = f.sc->dt[kp.cur], dt1 = f.sc->dt[kp.nxt];
+  const bool needH = PP || o.limiter >= MHY_LIM_MINMODG;
+  const bool mixteOpt = PP && o.pp_mixte == 1;
+  const bool pb = PP && o.methode == 2;
+
+  // one cell of this lane's line at along index b: Phi, centroid, width, flags
+  auto loadCell = [&](long b, double* ph, double& x, double& h, int& flg) {
+    const long jb = XD ? t : b, ib = XD ? b : t;
+    const long cb = jb * nx + ib;
+    double u[NC];
+    const int nload = PP ? 3 * M : NC;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cb] : 0.0;
+    cellPhi<M, PP>(u, PP ? 1.0 : f.VL[cb], PP ? f.qL[cb] : 0.0, ph);
+    x = XD ? f.XcL[cb] : f.YcL[cb];
+    h = needH ? widthW<XD>(kp, dt1, jb, ib) : 0.0;
+    flg = 0;
+    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<M>(ph);
+  };
+  double phi[NC], xc = 0.0, H = 0.0;
+  int fl = 0;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) phi[q] = 0.0;
+  if (in) loadCell(a, phi, xc, H, fl);
+
+  // neighbours along the sweep from the adjacent lanes; the halo lanes at
+  // the ends of the strip read their outer neighbour from memory; a missing
+  // neighbour takes this cell's values (PhaseRemap1.cc:157-158)
+  const int lm = (lane - S >= 0) ? lane - S : lane, lp = (lane + S < kWave) ? lane + S : lane;
+  const bool inM = (a >= 1), inP = (a + 1 < na);
+  double phm[NC], php[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const double vm = shfl(phi[q], lm), vp = shfl(phi[q], lp);
+    phm[q] = inM ? vm : phi[q];
+    php[q] = inP ? vp : phi[q];
+  }
+  double xcm = shfl(xc, lm), xcp = shfl(xc, lp);
+  xcm = inM ? xcm : xc, xcp = inP ? xcp : xc;
+  double Hm = shfl(H, lm), Hp = shfl(H, lp);
+  Hm = inM ? Hm : H, Hp = inP ? Hp : H;
+  int fm = shfl(fl, lm), fp = shfl(fl, lp);
+  fm = inM ? fm : fl, fp = inP ? fp : fl;
+  if (in && slot == 0 && inM) loadCell(a - 1, phm, xcm, Hm, fm);
+  if (in && slot == lastSlot && inP) loadCell(a + 1, php, xcp, Hp, fp);
+  Nbr n{phm, phi, php, xcm, xc, xcp, inM, inP, H, Hm, Hp, false};
+  if (mixteOpt) n.vp = ((fl >> 8) == 0) && fl == fp && fl == fm;
+
+  // normal velocities of the minus / plus faces (PrepareRemap.cc:50-62)
+  auto fnv = [&](long b) {
+    long n1, n2;
+    if (XD) n1 = j * (nx + 1) + b, n2 = (j + 1) * (nx + 1) + b;
+    else n1 = b * (nx + 1) + i, n2 = b * (nx + 1) + i + 1;
+    const double* uu = XD ? f.ux[kp.nxt] : f.uy[kp.nxt];
+    return 0.0 + ((0.0 + uu[n1]) + uu[n2]) * 0.5;
+  };
+  const double vm = in ? fnv(a) : 0.0, vp = in ? fnv(a + 1) : 0.0;
+  // Euler lengths of the minus / plus faces (Init.cc:528-533)
+  auto faceLen = [&](long b) {
+    D2 e1, e2;
+    if (XD) e1 = eulerNode(o, j, b), e2 = eulerNode(o, j + 1, b);
+    else e1 = eulerNode(o, b, i), e2 = eulerNode(o, b, i + 1);
+    return d2norm(d2sub(e2, e1));
+  };
+  const double Lm = faceLen(a), Lp = faceLen(a + 1);
+  const bool hasM = inM, hasP = inP;
+
+  double phiM[NC], phiP[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) phiM[q] = phiP[q] = 0.0;
+  if (!PP) {
+    // own limited slope, then the upwind value of each face
+    // (UtilesRemap-Impl.h:491-501); back/front cells: x: left/right,
+    // y: above/below
+    double g[NC], gm[NC], gp[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) g[q] = in ? gradQ<XD>(o, n, q) : 0.0;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) gm[q] = shfl(g[q], lm), gp[q] = shfl(g[q], lp);
+    // face between along-cells lo (lower index) and hi
+    auto faceValue = [&](long b, const double* pLo, const double* gLo, double xLo, const double* pHi,
+                         const double* gHi, double xHi, double v, double* out) {
+      const double dxl = XD ? xHi - xLo : xLo - xHi;  // front - back
+      const bool upLo = XD ? (v * dxl > 0.0) : !(v * dxl > 0.0);
+      const double xf = XD ? 0.5 * (o.dx * (double)b + o.dx * (double)b) : 0.5 * (o.dy * (double)b + o.dy * (double)b);
+      const double s = 0.0 + (xf - (upLo ? xLo : xHi));
+#pragma unroll
+      for (int q = 0; q < NC; ++q) out[q] = upLo ? pLo[q] + gLo[q] * s : pHi[q] + gHi[q] * s;
+    };
+    if (hasM) faceValue(a, phm, gm, xcm, phi, g, xc, vm, phiM);
+    if (hasP) faceValue(a + 1, phi, g, xc, php, gp, xcp, vp, phiP);
+  } else {
+    // PenteBorne: phi_f = Av(minus cell) - Ar(plus cell) (PhaseRemap1.cc:361-364)
+    double Ar[NC], Av[NC], Dual[M];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) Ar[q] = Av[q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) Dual[k] = 0.0;
+    const bool formed = o.order > 1;  // the fluxes are only formed for order > 1
+    if (formed && in) ppFluxes<M, XD>(o, n, vm, vp, dtn, pb && output, Ar, Av, Dual);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const double avm = shfl(Av[q], lm), arp = shfl(Ar[q], lp);
+      if (formed) {
+        phiM[q] = hasM ? avm - Ar[q] : 0.0;
+        phiP[q] = hasP ? Av[q] - arp : 0.0;
+      }
+    }
+    if (pb && output) {
+      const double w = (Lm + Lp) * 0.5;
+#pragma unroll
+      for (int k = 0; k < M; ++k) f.DF[k][c] = formed ? Dual[k] * w : 0.0;
+    }
+  }
+  if (!output) return;
+
+  // computeUremap (PhaseRemap1.cc:468-503): a cell receives -G through its
+  // minus face and +G through its plus face
+  const double coefM = PP ? Lm : vm * Lm * dtn;
+  const double coefP = PP ? Lp : vp * Lp * dtn;
+  double Gm[NC], red[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    Gm[q] = coefM * phiM[q];
+    const double Gp = coefP * phiP[q];
+    red[q] = 0.0;
+    if (hasM) red[q] = red[q] + (-Gm[q]);
+    if (hasP) red[q] = red[q] + Gp;
+  }
+  if (o.flux_any) {
+    // remap/BoundaryFluxes.cc:14-120: flux through one face of the cell with
+    // the face value of the opposite face; the first matching rule applies
+    int rule = 0;  // 1: through the minus face, 2: through the plus face
+    if (!XD && o.flux_bottom == 1 && c < nx) rule = 1;
+    else if (!XD && o.flux_top == 1 && c <= nx * (ny - 1) && c < nx * ny) rule = 2;
+    else if (XD && o.flux_left == 1 && i == 0) rule = 1;
+    else if (XD && o.flux_right == 1 && i == nx - 1) rule = 2;
+    if (rule != 0) {
+      const bool viaMinus = (rule == 1);
+      const bool oppOk = viaMinus ? hasP : hasM;
+      const double vb = viaMinus ? vm : vp;
+      const double Lb = viaMinus ? Lm : Lp;
+      const double sgn = viaMinus ? -1.0 : 1.0;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        const double ph = oppOk ? (viaMinus ? phiP[q] : phiM[q]) : 0.0;
+        const double Fb = PP ? (sgn * Lb) * ph : (((sgn * vb) * Lb) * dtn) * ph;
+        red[q] = red[q] + Fb;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) Uout[q][c] = Uin[q][c] - red[q];
+
+  // face fluxes for the dual remap (A1: mass, A2: volume): the minus face of
+  // each cell, plus the last face of the line
+  if (o.methode == 0 || o.methode == 1) {
+    const int q0 = (o.methode == 0) ? M : 0;
+    const long fm = XD ? j * (nx + 1) + a : a * nx + i;
+#pragma unroll
+    for (int k = 0; k < M; ++k) f.G[k][fm] = hasM ? Gm[q0 + k] : 0.0;
+    if (!hasP) {
+      const long fp = XD ? j * (nx + 1) + a + 1 : (a + 1) * nx + i;
+#pragma unroll
+      for (int k = 0; k < M; ++k) f.G[k][fp] = 0.0;
+    }
+  }
+}
+
+#define MHY_SWEEP_WAVE_INST(M, PP)                       \
+  template __global__ void k_sweep_wave<M, PP, true>(KP); \
+  template __global__ void k_sweep_wave<M, PP, false>(KP);
+MHY_SWEEP_WAVE_INST(1, false)
+MHY_SWEEP_WAVE_INST(1, true)
+MHY_SWEEP_WAVE_INST(2, false)
+MHY_SWEEP_WAVE_INST(2, true)
+MHY_SWEEP_WAVE_INST(3, false)
+MHY_SWEEP_WAVE_INST(3, true)
+#undef MHY_SWEEP_WAVE_INST
+
+// launch grid of the wave sweeps (256-thread blocks of 4 waves)
+dim3 sweepWaveGrid(bool xd, long nx, long ny) {
+  if (xd) {
+    const long segs = (nx + kXOut - 1) / kXOut;
+    return dim3((unsigned)((segs + 3) / 4), (unsigned)ny);
+  }
+  return dim3((unsigned)((nx + 4 * kYCols - 1) / (4 * kYCols)), (unsigned)((ny + kYOut - 1) / kYOut));
+}
+
+}  // namespace mhy
