@@ -41,8 +41,8 @@ struct mhy_ctx {
   bool x_then_y_n = true;
   int iteration = 0;
   bool finished = false;
-  // sweep kernels: wave-shuffle (default) or per-cell stencil
-  // (environment MAHYCO_SWEEP_KERNEL=cell selects the latter)
+  // sweep and dual kernels: wave-shuffle (default) or per-cell/per-node
+  // stencil (environment MAHYCO_SWEEP_KERNEL=cell selects the latter)
   bool sweepWave = true;
   int status = MHY_OK;
   long ncell = 0, nnode = 0, nface = 0;
@@ -213,6 +213,11 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
     const KernelFn k = sweepKernel(x->sweepWave, xd, x->o.nbmat, x->o.pp == 1);
     const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny) : gridFor(x->ncell);
     timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, x->stream, kp); });
+  } else if (x->sweepWave && xd) {
+    // along y the per-node kernel is faster (a 2-column wave strip reads
+    // 16-byte row segments)
+    const dim3 grid = dualWaveGrid(true, x->o.nx, x->o.ny);
+    timed(x, K_DUAL_X, [&] { hipLaunchKernelGGL(k_dual_wave<true>, grid, dim3(kBlock), 0, x->stream, kp); });
   } else if (xd) {
     timed(x, K_DUAL_X, [&] { hipLaunchKernelGGL(k_dual<true>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
   } else {

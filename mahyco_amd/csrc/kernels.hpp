@@ -17,6 +17,9 @@ __global__ void k_sweep_wave(KP kp);
 dim3 sweepWaveGrid(bool xd, long nx, long ny);
 template <bool XD>
 __global__ void k_dual(KP kp);
+template <bool XD>
+__global__ void k_dual_wave(KP kp);
+dim3 dualWaveGrid(bool xd, long nx, long ny);
 __global__ void k_remap_final(KP kp);
 __global__ void k_node_mass(KP kp);
 __global__ void k_conservation(KP kp, double* out);

@@ -220,6 +220,206 @@ __global__ __launch_bounds__(256) void k_dual(KP kp) {
 template __global__ void k_dual<true>(KP);
 template __global__ void k_dual<false>(KP);
 
+// The same nodal remap with one lane per node and the neighbour values taken
+// by wave shuffles.  A wave covers a strip of nodes along the sweep: along x,
+// 64 nodes of a node row; along y, 2 node columns x 32 node rows.  A lane
+// needs its neighbours' gradients, which need nodes two further away, so the
+// 3 strip nodes at each end are halo (58 of 64, resp. 26 of 32 nodes are
+// updated).  Each lane computes its own DualPhi, Lagrange coordinate and
+// limited gradient once.
+namespace {
+constexpr int kDW = 64;
+constexpr int kDYCols = 2;
+constexpr int kDYRows = kDW / kDYCols;
+constexpr int kDXOut = kDW - 6;
+constexpr int kDYOut = kDYRows - 6;
+__device__ __forceinline__ double dshfl(double v, int src) { return __shfl(v, src, kDW); }
+}  // namespace
+
+template <bool XD>
+__global__ __launch_bounds__(256) void k_dual_wave(KP kp) {
+  constexpr int S = XD ? 1 : kDYCols;  // lane distance of the along-sweep neighbour
+  const Opts& o = kp.o;
+  const Fields& f = kp.f;
+  const int m = o.nbmat;
+  const long nx = o.nx, ny = o.ny;
+  const int lane = threadIdx.x & (kDW - 1), wave = threadIdx.x >> 6;
+  long a, t, na, nt;
+  int slot, last;
+  if (XD) {
+    const long seg = (long)blockIdx.x * 4 + wave;
+    slot = lane, last = kDW - 1;
+    a = seg * kDXOut - 3 + lane;
+    t = blockIdx.y;
+    na = nx, nt = ny;  // node along index in [0, na], across in [0, nt]
+  } else {
+    slot = lane / kDYCols, last = kDYRows - 1;
+    a = (long)blockIdx.y * kDYOut - 3 + slot;
+    t = (long)blockIdx.x * (4 * kDYCols) + wave * kDYCols + (lane % kDYCols);
+    na = ny, nt = nx;
+  }
+  const bool in = (a >= 0 && a <= na && t <= nt);
+  const long j = XD ? t : a, i = XD ? a : t;
+  const long p = j * (nx + 1) + i;
+  const bool output = in && slot >= 3 && slot <= last - 3;
+  const double dtn = f.sc->dt[kp.cur];
+  const double dt1 = f.sc->dt[kp.nxt];
+  const int order2 = o.order - 1;
+
+  // own DualPhi and Lagrange coordinate
+  double dpx0 = 0.0, dpy0 = 0.0, xl0 = 0.0;
+  if (in) {
+    if (kp.sweep == 1) {
+      const double ux1 = f.ux[kp.nxt][p], uy1 = f.uy[kp.nxt][p];
+      dpx0 = ux1, dpy0 = uy1;
+      const D2 e = eulerNode(o, j, i);
+      xl0 = XD ? e.x + ux1 * dt1 : e.y + uy1 * dt1;
+      // Lagrange coordinate along the second sweep's axis (the second sweep
+      // overwrites the velocity in place)
+      if (output) f.XL2[p] = XD ? e.y + uy1 * dt1 : e.x + ux1 * dt1;
+    } else {
+      const double M1 = f.UD1[2][p];
+      dpx0 = f.UD1[0][p] / M1, dpy0 = f.UD1[1][p] / M1;
+      xl0 = f.XL2[p];
+    }
+  }
+  // nodes a-2 .. a+2 (slots 1..5 of the reference stencil a-3 .. a+3)
+  double dpx[7], dpy[7], xl[7];
+  bool inS[7];
+#pragma unroll
+  for (int d = 1; d <= 5; ++d) {
+    const int off = d - 3;
+    const int src = lane + off * S;
+    const int srcC = (src >= 0 && src < kDW) ? src : lane;
+    dpx[d] = dshfl(dpx0, srcC), dpy[d] = dshfl(dpy0, srcC), xl[d] = dshfl(xl0, srcC);
+    const long b = a + off;
+    inS[d] = (b >= 0 && b <= na);
+    if (!inS[d]) dpx[d] = dpy[d] = xl[d] = 0.0;
+  }
+  dpx[3] = dpx0, dpy[3] = dpy0, xl[3] = xl0;
+  inS[0] = (a - 3 >= 0), inS[6] = (a + 3 <= na);
+  dpx[0] = dpy[0] = xl[0] = dpx[6] = dpy[6] = xl[6] = 0.0;
+
+  // own limited nodal gradient (UtilesRemap-Impl.h:8-97)
+  double g0[2] = {0.0, 0.0};
+  if (in && o.order > 1 && inS[2] && inS[4]) {
+    const int d = 3;
+    const double gr0 = (dpx[d + 1] - dpx[d]) / (xl[d + 1] - xl[d]);
+    const double gl0 = (dpx[d] - dpx[d - 1]) / (xl[d] - xl[d - 1]);
+    const double gr1 = (dpy[d + 1] - dpy[d]) / (xl[d + 1] - xl[d]);
+    const double gl1 = (dpy[d] - dpy[d - 1]) / (xl[d] - xl[d - 1]);
+    const double h0 = 0.5 * (xl[d + 1] - xl[d - 1]);
+    double hm, hp;
+    if (!inS[d - 2]) {
+      hm = 0., hp = 0.5 * (xl[d + 2] - xl[d]);
+    } else if (!inS[d + 2]) {
+      hp = 0., hm = 0.5 * (xl[d] - xl[d - 2]);
+    } else {
+      hm = 0.5 * (xl[d] - xl[d - 2]);
+      hp = 0.5 * (xl[d + 2] - xl[d]);
+    }
+    g0[0] = limitedGradient(o.limiter, gr0, gl0, dpx[d], dpx[d + 1], dpx[d - 1], h0, hp, hm);
+    g0[1] = limitedGradient(o.limiter, gr1, gl1, dpy[d], dpy[d + 1], dpy[d - 1], h0, hp, hm);
+  }
+  const int lmS = (lane - S >= 0) ? lane - S : lane, lpS = (lane + S < kDW) ? lane + S : lane;
+  double gM[2], gP[2];
+  gM[0] = dshfl(g0[0], lmS), gM[1] = dshfl(g0[1], lmS);
+  gP[0] = dshfl(g0[0], lpS), gP[1] = dshfl(g0[1], lpS);
+  if (!output) return;
+  // a neighbour outside the domain has a zero gradient
+  if (!inS[2]) gM[0] = gM[1] = 0.0;
+  if (!inS[4]) gP[0] = gP[1] = 0.0;
+
+  // nodal mass fluxes through the dual faces on the minus / plus side
+  double plus = 0., minus = 0.;
+  {
+    const bool interior = (a > 0 && a < na);
+    long tf[2];
+    int nf = 0;
+    if (interior) {
+      if (t >= 1) tf[nf++] = t - 1;
+      if (t <= nt - 1) tf[nf++] = t;
+    }
+    auto gidx = [&](long b, long tt) { return XD ? tt * (nx + 1) + b : b * nx + tt; };
+    auto cidx = [&](long b, long tt) { return XD ? tt * nx + b : b * nx + tt; };
+    for (int k = 0; k < m; ++k) {
+      double pk = 0, mk = 0;
+      if (o.methode == 0 || o.methode == 1) {
+        for (int s2 = 0; s2 < nf; ++s2) {
+          const long tt = tf[s2];
+          const double Gf = f.G[k][gidx(a, tt)];
+          double tp = Gf + f.G[k][gidx(a + 1, tt)];
+          double tm = Gf + f.G[k][gidx(a - 1, tt)];
+          if (o.methode == 1) tp = f.rL[cidx(a, tt)] * tp, tm = f.rL[cidx(a - 1, tt)] * tm;
+          pk += tp;
+          mk += tm;
+        }
+        const int nbfaces = 2 * nf;
+        if (nbfaces != 0) pk /= nbfaces, mk /= nbfaces;
+      } else if (o.methode == 2) {
+        int nbcell = 0;
+        for (int s2 = 0; s2 < nf; ++s2) {
+          const long tt = tf[s2];
+          pk += f.DF[k][cidx(a, tt)];
+          mk += f.DF[k][cidx(a - 1, tt)];
+          nbcell++;
+        }
+        if (nbcell != 0) pk /= nbcell, mk /= nbcell;
+      }
+      plus += pk;
+      minus += mk;
+    }
+  }
+  // upwind velocities (UtilesDualRemap.cc:234-397); a missing neighbour
+  // reads as 0 and its mass flux is 0
+  auto upwind = [&](bool minusSide, double flux, const double* gNb, double* r) {
+    const int dn = minusSide ? 2 : 4;
+    const double dNb[2] = {dpx[dn], dpy[dn]}, d0[2] = {dpx[3], dpy[3]};
+    const double uN = XD ? dpx[dn] : dpy[dn], u0 = XD ? dpx[3] : dpy[3];
+    const double u = minusSide ? 0.5 * (uN + u0) : 0.5 * (u0 + uN);
+    const int sign = (u > 0) ? 1 : -1;
+    const double h = minusSide ? (xl[3] - xl[dn]) : (xl[dn] - xl[3]);
+    const bool fromNb = minusSide ? (flux > 0) : (flux < 0);
+    for (int d = 0; d < 2; ++d) {
+      const double base = fromNb ? dNb[d] : d0[d];
+      const double g = fromNb ? gNb[d] : g0[d];
+      r[d] = base + order2 * (0.5 * g * (sign * h - dtn * u));
+    }
+  };
+  double uvm[2], uvp[2];
+  upwind(true, minus, gM, uvm);
+  upwind(false, plus, gP, uvp);
+  double U0, U1, U2;
+  if (kp.sweep == 1) {
+    const double Mp = f.Mp[p];
+    U0 = Mp * f.ux[kp.nxt][p], U1 = Mp * f.uy[kp.nxt][p], U2 = Mp;
+  } else {
+    U0 = f.UD1[0][p], U1 = f.UD1[1][p], U2 = f.UD1[2][p];
+  }
+  const double R2 = U2 + minus - plus;
+  const double R0 = U0 + minus * uvm[0] - plus * uvp[0];
+  const double R1 = U1 + minus * uvm[1] - plus * uvp[1];
+  if (kp.sweep == 1) {
+    f.UD1[0][p] = R0, f.UD1[1][p] = R1, f.UD1[2][p] = R2;
+  } else {
+    // remapVariables, nodes (PhaseRemapFinal.cc:207-234)
+    f.ux[kp.nxt][p] = R0 / R2;
+    f.uy[kp.nxt][p] = R1 / R2;
+  }
+}
+
+template __global__ void k_dual_wave<true>(KP);
+template __global__ void k_dual_wave<false>(KP);
+
+// launch grid of the wave dual kernels (256-thread blocks of 4 waves)
+dim3 dualWaveGrid(bool xd, long nx, long ny) {
+  if (xd) {
+    const long segs = (nx + 1 + kDXOut - 1) / kDXOut;
+    return dim3((unsigned)((segs + 3) / 4), (unsigned)(ny + 1));
+  }
+  return dim3((unsigned)((nx + 1 + 4 * kDYCols - 1) / (4 * kDYCols)), (unsigned)((ny + 1 + kDYOut - 1) / kDYOut));
+}
+
 // Vnr::remapVariables, cells (PhaseRemapFinal.cc:23-205), then computeEOS
 // and computePressionMoyenne (PhaseLagrange.cc:392-485) on the remapped state.
 __global__ __launch_bounds__(256) void k_remap_final(KP kp) {
