@@ -126,6 +126,7 @@ struct KP {
   int cur, nxt;  // parity of the n and n+1 buffers
   int sweep;     // 1 or 2
   int xdir;      // 1: sweep along x, 0: along y
+  int fuseFinal; // sweep 2 finishes each cell (final remap) instead of storing U2
   long ncell, nnode;
 };
 

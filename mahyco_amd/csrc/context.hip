@@ -44,6 +44,10 @@ struct mhy_ctx {
   // sweep and dual kernels: wave-shuffle (default) or per-cell/per-node
   // stencil (environment MAHYCO_SWEEP_KERNEL=cell selects the latter)
   bool sweepWave = true;
+  // sweep 2 runs the cell part of the final remap (environment
+  // MAHYCO_FUSE_FINAL=0 keeps it a separate kernel); finalDone records that
+  // the current iteration's final remap is already done
+  bool fuseFinal = true, finalDone = false;
   int status = MHY_OK;
   long ncell = 0, nnode = 0, nface = 0;
   std::string err;
@@ -109,6 +113,7 @@ KP makeKP(const mhy_ctx* x) {
   kp.nxt = 1 - x->cur;
   kp.sweep = 1;
   kp.xdir = 1;
+  kp.fuseFinal = 0;
   kp.ncell = x->ncell;
   kp.nnode = x->nnode;
   return kp;
@@ -207,6 +212,11 @@ KernelFn sweepKernel(bool wave, bool xd, int m, bool pp) {
 int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   KP kp = makeKP(x);
   kp.sweep = sweep;
+  if (stage == MHY_STAGE_CELLS) {
+    // the cell stage of sweep 2 also runs the cell part of remapVariables
+    kp.fuseFinal = (sweep == 2 && x->fuseFinal) ? 1 : 0;
+    x->finalDone = (kp.fuseFinal == 1);
+  }
   const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
   kp.xdir = xd ? 1 : 0;
   if (stage == MHY_STAGE_CELLS) {
@@ -233,8 +243,14 @@ int phaseSweep(mhy_ctx* x, int sweep) {
 }
 
 int phaseFinal(mhy_ctx* x) {
+  if (x->finalDone) {  // already done by the cell stage of sweep 2
+    x->finalDone = false;
+    return MHY_OK;
+  }
   KP kp = makeKP(x);
-  timed(x, K_FINAL, [&] { hipLaunchKernelGGL(k_remap_final, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+  static const KernelFn kf[3] = {k_remap_final<1>, k_remap_final<2>, k_remap_final<3>};
+  const KernelFn k = kf[x->o.nbmat - 1];
+  timed(x, K_FINAL, [&] { hipLaunchKernelGGL(k, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
   return checkLaunch(x, "final");
 }
 
@@ -325,6 +341,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   mhy_ctx* x = new mhy_ctx();
   x->cfg = *cfg;
   if (const char* sk = std::getenv("MAHYCO_SWEEP_KERNEL")) x->sweepWave = (std::string(sk) != "cell");
+  if (const char* ff = std::getenv("MAHYCO_FUSE_FINAL")) x->fuseFinal = (std::string(ff) != "0");
   const mhy_config& c = *cfg;
   auto bail = [&](int code) {
     // keep the context so the caller can read the message, unless it is

@@ -1,0 +1,143 @@
+// Final remap of one cell (Vnr::remapVariables, cells: PhaseRemapFinal.cc:23-205)
+// followed by computeEOS and computePressionMoyenne (PhaseLagrange.cc:392-485)
+// on the remapped state.  Shared by the stand-alone final-remap kernel and by
+// the second remap sweep, which finishes each cell from its registers.
+#pragma once
+
+#include "common.hpp"
+
+namespace mhy {
+
+// first error wins (MHY_ERR_*), read by the host after the step
+__device__ __forceinline__ void raiseErr(Scalars* sc, int code) { atomicCAS(&sc->err, 0, code); }
+
+// U: the 3m+1 live components of the remapped vector U2 of cell c
+template <int M>
+__device__ __forceinline__ void finalCell(const KP& kp, long c, const double* U) {
+  const Opts& o = kp.o;
+  const Fields& f = kp.f;
+  const int cur = kp.cur, nxt = kp.nxt;
+  const double thr = o.threshold;
+  const double vol = o.dx * o.dy;  // m_euler_volume (Init.cc:427)
+  double volt = 0., masset = 0.;
+  double vn[M], fr[M], mu[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    vn[k] = U[k];
+    volt += vn[k];
+    masset += U[M + k];
+  }
+  double vtn = 0.;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    vn[k] *= vol / volt;
+    vtn += vn[k];
+  }
+  double sf = 0.;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    fr[k] = vn[k] / vtn;
+    if (fr[k] < thr) fr[k] = 0.;
+    sf += fr[k];
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) fr[k] = fr[k] / sf;
+  int matcell = 0, imatpure = -1;
+#pragma unroll
+  for (int k = 0; k < M; ++k)
+    if (fr[k] > 0.) matcell++, imatpure = k;
+  const int mixte = (matcell > 1) ? 1 : 0;
+  const int pure = (matcell > 1) ? -1 : imatpure;
+#pragma unroll
+  for (int k = 0; k < M; ++k) mu[k] = U[M + k] / masset;
+  double fm = 0.;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    if (fr[k] < thr) mu[k] = 0.;
+    fm += mu[k];
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) mu[k] /= fm;
+  double dens[M], pe[M];
+  double dn = 0.;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    dens[k] = 0.0;
+    if (fr[k] > thr) dens[k] = U[M + k] / vn[k];
+    dn += fr[k] * dens[k];
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    pe[k] = 0.0;
+    if ((fr[k] > thr) && (U[M + k] != 0.)) pe[k] = U[2 * M + k] / U[M + k];
+  }
+  const double rho1 = dn;
+  double e1 = 0.;
+  const double Mc = vol * rho1;
+  double Mk[M], rk[M], ek[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) Mk[k] = mu[k] * Mc;
+  const double q1 = U[3 * M] / vol;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    rk[k] = dens[k];
+    ek[k] = pe[k];
+    e1 += mu[k] * ek[k];
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    if (pe[k] < 0. || dens[k] < 0.) rk[k] = 0., ek[k] = 0., mu[k] = 0., fr[k] = 0.;
+  }
+  if (e1 != e1 || rho1 != rho1) raiseErr(f.sc, MHY_ERR_NAN);
+  // computeEOS on the remapped state
+  double pk[M], ck[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const double g = o.gammap[k];
+    if (o.eos[k] == MHY_EOS_PERFECT_GAS) {
+      pk[k] = (g - 1.0) * rk[k] * ek[k];
+      ck[k] = sqrt(g * (g - 1.0) * ek[k]);
+    } else if (o.eos[k] == MHY_EOS_VOID) {
+      pk[k] = f.p_k[cur][k][c];
+      ck[k] = f.c_k[cur][k][c];
+      ek[k] = f.e_k[cur][k][c];
+      e1 = 0.0;
+    } else {
+      pk[k] = f.p_k[nxt][k][c];
+      ck[k] = f.c_k[nxt][k][c];
+    }
+  }
+  // computePressionMoyenne
+  double p1 = 0.0, c1 = 0.0;
+  const bool needStale = !(rho1 > 0.);
+  if (needStale) c1 = f.c[nxt][c];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    p1 += fr[k] * pk[k];
+    if (needStale) c1 = smax(c1, ck[k]);
+  }
+  if (rho1 > 0.) c1 = sqrt(o.gammap[0] * (o.gammap[0] - 1.0) * e1);
+#pragma unroll
+  for (int k = 0; k < M; ++k)
+    if (o.eos[k] == MHY_EOS_VOID) e1 += mu[k] * ek[k];
+  f.rho[nxt][c] = rho1;
+  f.e[nxt][c] = e1;
+  f.q[nxt][c] = q1;
+  f.p[nxt][c] = p1;
+  f.c[nxt][c] = c1;
+  f.M[c] = Mc;
+  f.mixte[c] = mixte;
+  f.pure[c] = pure;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    f.f_k[k][c] = fr[k];
+    f.mu_k[k][c] = mu[k];
+    f.M_k[k][c] = Mk[k];
+    f.rho_k[nxt][k][c] = rk[k];
+    f.e_k[nxt][k][c] = ek[k];
+    f.p_k[nxt][k][c] = pk[k];
+    f.c_k[nxt][k][c] = ck[k];
+  }
+}
+
+}  // namespace mhy

@@ -20,6 +20,7 @@ __global__ void k_dual(KP kp);
 template <bool XD>
 __global__ void k_dual_wave(KP kp);
 dim3 dualWaveGrid(bool xd, long nx, long ny);
+template <int M>
 __global__ void k_remap_final(KP kp);
 __global__ void k_node_mass(KP kp);
 __global__ void k_conservation(KP kp, double* out);

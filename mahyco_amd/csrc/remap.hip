@@ -5,13 +5,12 @@
 // Layout: structure-of-arrays FP64 in HBM, one thread per node (cell) with
 // the stencil along the sweep read through the L1/L2 caches.
 #include "common.hpp"
+#include "final_math.hpp"
 #include "kernels.hpp"
 
 namespace mhy {
 
 namespace {
-
-__device__ __forceinline__ void raise(Scalars* sc, int code) { atomicCAS(&sc->err, 0, code); }
 
 // remap/UtilesRemap.cc:9-23
 __device__ __forceinline__ double fluxLimiter(int lim, double r) {
@@ -422,120 +421,19 @@ dim3 dualWaveGrid(bool xd, long nx, long ny) {
 
 // Vnr::remapVariables, cells (PhaseRemapFinal.cc:23-205), then computeEOS
 // and computePressionMoyenne (PhaseLagrange.cc:392-485) on the remapped state.
+template <int M>
 __global__ __launch_bounds__(256) void k_remap_final(KP kp) {
-  const Opts& o = kp.o;
-  const Fields& f = kp.f;
+  constexpr int NC = 3 * M + 1;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= kp.ncell) return;
-  const int m = o.nbmat;
-  const int cur = kp.cur, nxt = kp.nxt;
-  const double thr = o.threshold;
-  double U[NCMAX];
-  for (int q = 0; q < o.nc; ++q) U[q] = f.U2[q][c];
-  const double vol = o.dx * o.dy;  // m_euler_volume (Init.cc:427)
-  double volt = 0., masset = 0.;
-  double vn[NMAT], fr[NMAT], mu[NMAT];
-  for (int k = 0; k < m; ++k) {
-    vn[k] = U[k];
-    volt += vn[k];
-    masset += U[m + k];
-  }
-  double vtn = 0.;
-  for (int k = 0; k < m; ++k) {
-    vn[k] *= vol / volt;
-    vtn += vn[k];
-  }
-  double sf = 0.;
-  for (int k = 0; k < m; ++k) {
-    fr[k] = vn[k] / vtn;
-    if (fr[k] < thr) fr[k] = 0.;
-    sf += fr[k];
-  }
-  for (int k = 0; k < m; ++k) fr[k] = fr[k] / sf;
-  int matcell = 0, imatpure = -1;
-  for (int k = 0; k < m; ++k)
-    if (fr[k] > 0.) matcell++, imatpure = k;
-  const int mixte = (matcell > 1) ? 1 : 0;
-  const int pure = (matcell > 1) ? -1 : imatpure;
-  for (int k = 0; k < m; ++k) mu[k] = U[m + k] / masset;
-  double fm = 0.;
-  for (int k = 0; k < m; ++k) {
-    if (fr[k] < thr) mu[k] = 0.;
-    fm += mu[k];
-  }
-  for (int k = 0; k < m; ++k) mu[k] /= fm;
-  double dens[NMAT], pe[NMAT];
-  double dn = 0.;
-  for (int k = 0; k < m; ++k) {
-    dens[k] = 0.0;
-    if (fr[k] > thr) dens[k] = U[m + k] / vn[k];
-    dn += fr[k] * dens[k];
-  }
-  for (int k = 0; k < m; ++k) {
-    pe[k] = 0.0;
-    if ((fr[k] > thr) && (U[m + k] != 0.)) pe[k] = U[2 * m + k] / U[m + k];
-  }
-  const double rho1 = dn;
-  double e1 = 0.;
-  const double M = vol * rho1;
-  double Mk[NMAT], rk[NMAT], ek[NMAT];
-  for (int k = 0; k < m; ++k) Mk[k] = mu[k] * M;
-  const double q1 = U[3 * m] / vol;
-  for (int k = 0; k < m; ++k) {
-    rk[k] = dens[k];
-    ek[k] = pe[k];
-    e1 += mu[k] * ek[k];
-  }
-  for (int k = 0; k < m; ++k) {
-    if (pe[k] < 0. || dens[k] < 0.) rk[k] = 0., ek[k] = 0., mu[k] = 0., fr[k] = 0.;
-  }
-  if (e1 != e1 || rho1 != rho1) raise(f.sc, MHY_ERR_NAN);
-  // computeEOS on the remapped state
-  double pk[NMAT], ck[NMAT];
-  for (int k = 0; k < m; ++k) {
-    const double g = o.gammap[k];
-    if (o.eos[k] == MHY_EOS_PERFECT_GAS) {
-      pk[k] = (g - 1.0) * rk[k] * ek[k];
-      ck[k] = sqrt(g * (g - 1.0) * ek[k]);
-    } else if (o.eos[k] == MHY_EOS_VOID) {
-      pk[k] = f.p_k[cur][k][c];
-      ck[k] = f.c_k[cur][k][c];
-      ek[k] = f.e_k[cur][k][c];
-      e1 = 0.0;
-    } else {
-      pk[k] = f.p_k[nxt][k][c];
-      ck[k] = f.c_k[nxt][k][c];
-    }
-  }
-  // computePressionMoyenne
-  double p1 = 0.0, c1 = 0.0;
-  const bool needStale = !(rho1 > 0.);
-  if (needStale) c1 = f.c[nxt][c];
-  for (int k = 0; k < m; ++k) {
-    p1 += fr[k] * pk[k];
-    if (needStale) c1 = smax(c1, ck[k]);
-  }
-  if (rho1 > 0.) c1 = sqrt(o.gammap[0] * (o.gammap[0] - 1.0) * e1);
-  for (int k = 0; k < m; ++k)
-    if (o.eos[k] == MHY_EOS_VOID) e1 += mu[k] * ek[k];
-  f.rho[nxt][c] = rho1;
-  f.e[nxt][c] = e1;
-  f.q[nxt][c] = q1;
-  f.p[nxt][c] = p1;
-  f.c[nxt][c] = c1;
-  f.M[c] = M;
-  f.mixte[c] = mixte;
-  f.pure[c] = pure;
-  for (int k = 0; k < m; ++k) {
-    f.f_k[k][c] = fr[k];
-    f.mu_k[k][c] = mu[k];
-    f.M_k[k][c] = Mk[k];
-    f.rho_k[nxt][k][c] = rk[k];
-    f.e_k[nxt][k][c] = ek[k];
-    f.p_k[nxt][k][c] = pk[k];
-    f.c_k[nxt][k][c] = ck[k];
-  }
+  double U[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) U[q] = kp.f.U2[q][c];
+  finalCell<M>(kp, c, U);
 }
+template __global__ void k_remap_final<1>(KP);
+template __global__ void k_remap_final<2>(KP);
+template __global__ void k_remap_final<3>(KP);
 
 // computeNodeMass for output (PhaseLagrange.cc:31-48)
 __global__ void k_node_mass(KP kp) {

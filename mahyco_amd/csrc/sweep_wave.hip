@@ -11,6 +11,7 @@
 // fluxes Ar and Av) once, and takes the neighbour values it needs with wave
 // shuffles: no LDS is allocated.  Operation order follows the reference, so
 // FP64 results are bitwise those of the CPU path.
+#include "final_math.hpp"
 #include "kernels.hpp"
 #include "sweep_math.hpp"
 
@@ -236,8 +237,15 @@ __global__ __launch_bounds__(256) void k_sweep_wave(KP kp) {
       }
     }
   }
+  double Unew[NC];
 #pragma unroll
-  for (int q = 0; q < NC; ++q) Uout[q][c] = Uin[q][c] - red[q];
+  for (int q = 0; q < NC; ++q) Unew[q] = Uin[q][c] - red[q];
+  if (kp.fuseFinal) {
+    finalCell<M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
+  } else {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) Uout[q][c] = Unew[q];
+  }
 
   // face fluxes for the dual remap (A1: mass, A2: volume): the minus face of
   // each cell, plus the last face of the line
