@@ -31,6 +31,10 @@ def workload(name):
         return "point explosion 4096x4096, 1 material, minmod order 2, FP64", configs.point_explosion()
     if name == "c4":
         return "three-material layout 3072x1024, minmod, FP64", configs.triple_layout()
+    if name.startswith("riemann@"):
+        n = int(name[8:])
+        c = configs.riemann(n, 0.45, 1.0, 1.0, 0.125, 0.1, 1.4, 1.67)
+        return f"two-material Riemann {n}x{n}, straight interface, minmod, FP64", c
     if name.startswith("c2@"):
         n = int(name[3:])
         c = configs.riemann_perturbed(n=n)
@@ -88,6 +92,49 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
 
 
+def bench_ensemble(args):
+    """C5: 512 two-material Riemann problems of uneven sizes stepped together
+    (one launch per kernel over all problems, HIP graphs), against the same
+    problems run one after another with the single-problem path."""
+    import mahyco_amd
+    from mahyco_amd import configs
+    cfgs = configs.ensemble(n=args.ensemble)
+    batch = mahyco_amd.Batch(cfgs)
+    cells = batch.cells
+    batch.run(max(args.warmup, 6))  # the first steps also tune the launch widths
+    t0 = time.perf_counter()
+    batch.run(args.steps)  # ends with a stream synchronisation
+    wall = time.perf_counter() - t0
+    batch.close()
+    value = cells * args.steps / wall
+    # the same problems, one at a time (bounded number of steps)
+    plain_steps = max(1, min(args.steps, args.plain_steps))
+    plain_t = 0.0
+    for c in cfgs:
+        ctx = mahyco_amd.Context(c)
+        ctx.run(1)
+        t1 = time.perf_counter()
+        ctx.run(plain_steps)
+        plain_t += time.perf_counter() - t1
+        ctx.close()
+    plain = cells * plain_steps / plain_t
+    B = algorithmic_bytes_per_cell(2)
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"ensemble of {len(cfgs)} two-material Riemann problems, sides 64..1024 "
+                               f"log-uniform (seed 20261204), minmod, FP64", "problems": len(cfgs), "cells": cells,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "whole step (batched)", "achieved": B * value / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B * value / HBM_PEAK, "traffic": None},
+        "plain_form": {"value": plain, "unit": "cell-updates/s", "steps": plain_steps,
+                       "note": "same problems, one context after another (mhy_run per problem)"},
+        "batched_over_plain": value / plain,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +144,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    ap.add_argument("--ensemble", type=int, default=512, help="problems of the c5 ensemble")
+    ap.add_argument("--plain-steps", type=int, default=5, help="steps of the one-by-one comparison (c5)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -109,6 +158,8 @@ def main():
 
     import mahyco_amd
 
+    if args.config == "c5":
+        return bench_ensemble(args)
     desc, cfg = workload(args.config)
     ctx = mahyco_amd.Context(cfg)
     cells = cfg.nx * cfg.ny

@@ -238,6 +238,22 @@ int mhy_set_field(mhy_ctx* ctx, int field, const double* in, long long n);
  * sums[2..4] = mass of material k; FP64 compensated device reductions. */
 int mhy_conservation(mhy_ctx* ctx, double* sums);
 
+/* Ensembles (no counterpart in the reference, which runs one problem per
+ * process): n independent VNR Lagrange + remap problems of any mesh sizes,
+ * sharing the material count and the PenteBorne option, stepped together.
+ * Each kernel of a step is one launch over all problems; every problem keeps
+ * its own arrays, dt and time, and its results are bitwise those of the same
+ * problem run alone.  mhy_batch_run(b, n) advances every problem by n steps
+ * (no per-problem stopping test); mhy_batch_context gives problem i's context
+ * for field access (owned by the batch: do not destroy it). */
+typedef struct mhy_batch mhy_batch;
+int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** batch);
+int mhy_batch_destroy(mhy_batch* batch);
+const char* mhy_batch_last_error(const mhy_batch* batch);
+int mhy_batch_run(mhy_batch* batch, int nsteps);
+int mhy_batch_context(mhy_batch* batch, int i, mhy_ctx** ctx);
+long long mhy_batch_cells(const mhy_batch* batch);
+
 /* Timing of the kernels launched by the last mhy_run (HIP events on the
  * context stream): per-kernel average milliseconds, by kernel index. */
 int mhy_kernel_count(const mhy_ctx* ctx);

@@ -22,6 +22,8 @@ EXPORTS = [
     "mhy_remap_final", "mhy_end_iteration", "mhy_synchronize", "mhy_get_time_state", "mhy_get_mesh_size",
     "mhy_get_field", "mhy_set_field", "mhy_conservation", "mhy_kernel_count", "mhy_kernel_name",
     "mhy_kernel_bytes", "mhy_enable_kernel_timing", "mhy_kernel_time",
+    "mhy_batch_create", "mhy_batch_destroy", "mhy_batch_last_error", "mhy_batch_run", "mhy_batch_context",
+    "mhy_batch_cells",
 ]
 
 _lib = None
@@ -67,6 +69,14 @@ def library():
         L.mhy_kernel_bytes.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.mhy_enable_kernel_timing.argtypes = [P, ctypes.c_int]
         L.mhy_kernel_time.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+        L.mhy_batch_create.argtypes = [ctypes.POINTER(MhyConfig), ctypes.c_int, ctypes.POINTER(P)]
+        L.mhy_batch_destroy.argtypes = [P]
+        L.mhy_batch_last_error.argtypes = [P]
+        L.mhy_batch_last_error.restype = ctypes.c_char_p
+        L.mhy_batch_run.argtypes = [P, ctypes.c_int]
+        L.mhy_batch_context.argtypes = [P, ctypes.c_int, ctypes.POINTER(P)]
+        L.mhy_batch_cells.argtypes = [P]
+        L.mhy_batch_cells.restype = ctypes.c_longlong
         _lib = L
     return _lib
 
@@ -167,3 +177,56 @@ class Context:
             out[L.mhy_kernel_name(self.h, k).decode()] = {"avg_ms": ms.value, "launches": n.value,
                                                          "bytes": b.value}
         return out
+
+
+class _BatchMember(Context):
+    """Problem i of a Batch: field and time-state access; owned by the batch."""
+
+    def __init__(self, batch, h, cfg):
+        self._batch = batch
+        self.cfg = cfg
+        self.nx, self.ny = cfg.nx, cfg.ny
+        self.h = h
+
+    def close(self):
+        self.h = None  # the batch owns the context
+
+
+class Batch:
+    """An ensemble of independent problems stepped together
+    (mhy_batch_create .. mhy_batch_destroy)."""
+
+    def __init__(self, cfgs):
+        self.cfgs = list(cfgs)
+        arr = (MhyConfig * len(self.cfgs))(*self.cfgs)
+        h = ctypes.c_void_p()
+        st = library().mhy_batch_create(arr, len(self.cfgs), ctypes.byref(h))
+        self.h = h
+        if st != MHY_OK:
+            msg = library().mhy_batch_last_error(h).decode() if h else ""
+            self.close()
+            raise MahycoError(f"mhy_batch_create failed ({st}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().mhy_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, n):
+        st = library().mhy_batch_run(self.h, n)
+        if st != MHY_OK:
+            raise MahycoError(f"mhy_batch_run failed ({st}): {library().mhy_batch_last_error(self.h).decode()}")
+
+    @property
+    def cells(self):
+        return library().mhy_batch_cells(self.h)
+
+    def problem(self, i):
+        h = ctypes.c_void_p()
+        st = library().mhy_batch_context(self.h, i, ctypes.byref(h))
+        if st != MHY_OK:
+            raise MahycoError(f"mhy_batch_context({i}) failed ({st})")
+        return _BatchMember(self, h, self.cfgs[i])

@@ -111,3 +111,22 @@ def riemann(n, x0, rho_l, p_l, rho_r, p_r, g_l, g_r):
     for i, v in enumerate([x0, rho_l, p_l, rho_r, p_r]):
         c.param[i] = v
     return c
+
+
+def ensemble(n=512, seed=20261204, side_min=64, side_max=1024):
+    """C5: n independent two-material Riemann problems on square meshes whose
+    side is drawn log-uniformly in [side_min, side_max]; random left/right
+    states (rho in [0.1,1], p in [0.05,1], gamma in {1.4, 1.67}) and interface
+    position in [0.3, 0.7]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    ratio = side_max / side_min
+    for _ in range(n):
+        side = int(round(side_min * ratio ** rng.uniform()))
+        rho_l, rho_r = rng.uniform(0.1, 1.0, 2)
+        p_l, p_r = rng.uniform(0.05, 1.0, 2)
+        g_l, g_r = rng.choice([1.4, 1.67], 2)
+        x0 = rng.uniform(0.3, 0.7)
+        out.append(riemann(side, float(x0), float(rho_l), float(p_l), float(rho_r), float(p_r), float(g_l), float(g_r)))
+    return out

@@ -130,6 +130,52 @@ struct KP {
   long ncell, nnode;
 };
 
+// One launch over an ensemble of problems: the launch parameters of problem
+// p for step variant v are table[v * nprob + p]; the virtual blocks of problem
+// p are start[p] .. start[p+1]-1, laid out as rows of gx[p] blocks.
+struct BatchLaunch {
+  const KP* table;
+  const unsigned* start;  // nprob + 1 entries
+  const unsigned* gx;     // nprob entries
+  int nprob, variant;
+  unsigned total;         // start[nprob]
+};
+
+// problem p and block index inside it of virtual block vb (binary search)
+__device__ __forceinline__ void batchLocate(const BatchLaunch& bl, unsigned vb, int& p, unsigned& local) {
+  int lo = 0, hi = bl.nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bl.start[mid] <= vb) lo = mid;
+    else hi = mid - 1;
+  }
+  p = lo;
+  local = vb - bl.start[lo];
+}
+
+// Runs fn(kp, local block, gx, blocks of the problem) for a contiguous range
+// of virtual blocks: workgroup w takes blocks [w*per, (w+1)*per), so that it
+// stays on one problem (and its launch parameters) for as long as possible.
+template <class F>
+__device__ __forceinline__ void batchForEach(const BatchLaunch& bl, F&& fn) {
+  const unsigned per = (bl.total + gridDim.x - 1) / gridDim.x;
+  unsigned vb = blockIdx.x * per;
+  const unsigned end = (vb + per < bl.total) ? vb + per : bl.total;
+  if (vb >= end) return;
+  int p;
+  unsigned local;
+  batchLocate(bl, vb, p, local);
+  unsigned pend = bl.start[p + 1];
+  for (; vb < end; ++vb, ++local) {
+    while (vb >= pend) {  // next problem (problems without blocks are skipped)
+      ++p;
+      local = 0;
+      pend = bl.start[p + 1];
+    }
+    fn(bl.table[bl.variant * bl.nprob + p], local, bl.gx[p], pend - bl.start[p]);
+  }
+}
+
 // Euler node coordinates (CartesianMesh2DGenerator.cc:52-55: x = dx*i, y = dy*j)
 MHY_HD D2 eulerNode(const Opts& o, long j, long i) { return {o.dx * (double)i, o.dy * (double)j}; }
 

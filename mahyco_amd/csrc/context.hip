@@ -480,6 +480,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     hipLaunchKernelGGL(k_node_mass, gridFor(N), dim3(kBlock), 0, x->stream, kp);
     if (hipStreamSynchronize(x->stream) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, "node mass"));
   }
+  x->init = HostInit();  // the host copy of the initial state is no longer needed
   x->cur = 0;
   x->x_then_y_n = true;
   x->iteration = 0;
@@ -790,3 +791,278 @@ int ctxNbmat(const mhy_ctx* x) { return x->o.nbmat; }
 int ctxProjection(const mhy_ctx* x) { return x->cfg.avec_projection; }
 void ctxMesh(const mhy_ctx* x, long& nx, long& ny) { nx = x->o.nx, ny = x->o.ny; }
 }  // namespace mhy
+
+// ------------------------------------------------------------------ batches
+// An ensemble of independent problems of any sizes, stepped together: every
+// kernel of a step is one launch over the virtual blocks of all problems
+// (BatchLaunch), each problem keeping its own arrays, dt and time.  The
+// number of workgroups of each launch is chosen by timing candidates over the
+// first steps (results do not depend on it); the step is then replayed from
+// two captured HIP graphs (even and odd steps).
+namespace {
+using namespace mhy;
+
+enum BatchKernel { BK_DT = 0, BK_NODE, BK_CELL, BK_SWX, BK_SWY, BK_DUX, BK_DUY, BK_COUNT };
+constexpr int kVariants = 16;  // cur (1 bit), sweep-1 (1), xdir (1), fuseFinal (1)
+constexpr unsigned kCand[] = {0u, 1024u, 2048u, 4096u, 8192u};  // 0: one workgroup per virtual block
+constexpr int kNCand = sizeof(kCand) / sizeof(kCand[0]);
+using BatchFn = void (*)(BatchLaunch);
+
+BatchFn sweepBatchKernel(bool xd, int m, bool pp) {
+  static const BatchFn t[2][3][2] = {
+      {{k_sweep_wave_batch<1, false, false>, k_sweep_wave_batch<1, true, false>},
+       {k_sweep_wave_batch<2, false, false>, k_sweep_wave_batch<2, true, false>},
+       {k_sweep_wave_batch<3, false, false>, k_sweep_wave_batch<3, true, false>}},
+      {{k_sweep_wave_batch<1, false, true>, k_sweep_wave_batch<1, true, true>},
+       {k_sweep_wave_batch<2, false, true>, k_sweep_wave_batch<2, true, true>},
+       {k_sweep_wave_batch<3, false, true>, k_sweep_wave_batch<3, true, true>}}};
+  return t[xd ? 1 : 0][m - 1][pp ? 1 : 0];
+}
+}  // namespace
+
+struct mhy_batch {
+  std::vector<mhy_ctx*> ctx;
+  hipStream_t stream = nullptr;
+  int m = 0;
+  bool pp = false;
+  KP* dTable = nullptr;
+  unsigned* dStart = nullptr;  // BK_COUNT x (n+1)
+  unsigned* dGx = nullptr;     // BK_COUNT x n
+  unsigned total[BK_COUNT] = {};
+  unsigned blocks[BK_COUNT] = {};  // workgroups per launch (0: total)
+  int cur = 0, cur0 = 0;
+  bool xty = true, xty0 = true;
+  long iteration = 0;
+  long long cells = 0;
+  bool tune = true, useGraph = true;
+  int tuneStep = 0;
+  float tuneMs[BK_COUNT][kNCand] = {};
+  hipEvent_t ev[2 * BK_COUNT] = {};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::string err;
+};
+
+namespace {
+
+int batchFail(mhy_batch* b, int code, const std::string& msg) {
+  b->err = msg;
+  return code;
+}
+
+// one step of every problem; timing != 0 records events around each launch
+int batchEnqueueStep(mhy_batch* b, int cur, bool xty, bool timing) {
+  const int n = (int)b->ctx.size();
+  auto launch = [&](int k, BatchFn fn, int variant) {
+    BatchLaunch bl{b->dTable, b->dStart + (size_t)k * (n + 1), b->dGx + (size_t)k * n, n, variant, b->total[k]};
+    const unsigned grid = b->blocks[k] ? std::min(b->blocks[k], b->total[k]) : b->total[k];
+    if (timing) (void)hipEventRecord(b->ev[2 * k], b->stream);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, b->stream, bl);
+    if (timing) (void)hipEventRecord(b->ev[2 * k + 1], b->stream);
+  };
+  launch(BK_DT, k_dt_reduce_batch, cur);
+  launch(BK_NODE, k_lagrange_node_batch, cur);
+  launch(BK_CELL, k_lagrange_cell_batch, cur);
+  for (int sweep = 1; sweep <= 2; ++sweep) {
+    const bool xd = (sweep == 1) ? xty : !xty;
+    const int fuse = (sweep == 2) ? 1 : 0;
+    const int v = cur + 2 * (sweep - 1) + 4 * (xd ? 1 : 0);
+    launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel(xd, b->m, b->pp), v + 8 * fuse);
+    if (xd) launch(BK_DUX, k_dual_wave_batch<true>, v);
+    else launch(BK_DUY, k_dual_batch<false>, v);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MHY_OK : batchFail(b, MHY_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+}
+
+// captures the two step parities of the lock-step ensemble as graphs:
+// exec[c] is the step that starts with parity cur = c
+int batchCapture(mhy_batch* b) {
+  for (int c = 0; c < 2; ++c) {
+    const bool xty = (c == b->cur0) ? b->xty0 : !b->xty0;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return batchFail(b, MHY_ERR_DEVICE, "graph capture begin");
+    int st = batchEnqueueStep(b, c, xty, false);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st != MHY_OK) return st;
+    if (e != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->exec[c], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+  }
+  return MHY_OK;
+}
+
+// picks, per kernel, the candidate workgroup count with the shortest time
+void batchPickBlocks(mhy_batch* b) {
+  for (int k = 0; k < BK_COUNT; ++k) {
+    int best = 0;
+    for (int c = 1; c < kNCand; ++c)
+      if (b->tuneMs[k][c] > 0.f && (b->tuneMs[k][best] <= 0.f || b->tuneMs[k][c] < b->tuneMs[k][best])) best = c;
+    b->blocks[k] = kCand[best];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
+  if (!cfgs || n <= 0 || !out) return MHY_ERR_ARG;
+  *out = nullptr;
+  mhy_batch* b = new mhy_batch();
+  *out = b;
+  for (int i = 0; i < n; ++i) {
+    const mhy_config& c = cfgs[i];
+    if (c.schema != MHY_SCHEMA_VNR || !c.avec_projection || c.sans_lagrange)
+      return batchFail(b, MHY_ERR_ARG, "batch problems must use the VNR Lagrange + remap mode");
+    if (i > 0 && (c.nbmat != cfgs[0].nbmat || c.pp != cfgs[0].pp))
+      return batchFail(b, MHY_ERR_ARG, "batch problems must share the material count and the PenteBorne option");
+  }
+  b->m = cfgs[0].nbmat;
+  b->pp = cfgs[0].pp == 1;
+  if (const char* g = std::getenv("MAHYCO_BATCH_GRAPH")) b->useGraph = (std::string(g) != "0");
+  if (const char* t = std::getenv("MAHYCO_BATCH_TUNE")) b->tune = (std::string(t) != "0");
+  b->ctx.resize(n, nullptr);
+  for (int i = 0; i < n; ++i) {
+    int st = mhy_create(&cfgs[i], &b->ctx[i]);
+    if (st != MHY_OK)
+      return batchFail(b, st, "problem " + std::to_string(i) + ": " + (b->ctx[i] ? mhy_last_error(b->ctx[i]) : ""));
+    b->ctx[i]->sweepWave = true;
+    b->ctx[i]->fuseFinal = true;
+    b->cells += b->ctx[i]->ncell;
+  }
+  b->cur0 = b->cur = b->ctx[0]->cur;
+  b->xty0 = b->xty = b->ctx[0]->x_then_y_n;
+  if (hipStreamCreate(&b->stream) != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, "stream");
+  for (auto& e : b->ev)
+    if (hipEventCreate(&e) != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, "events");
+  // launch parameters of every problem and step variant
+  std::vector<KP> table((size_t)kVariants * n);
+  for (int v = 0; v < kVariants; ++v)
+    for (int i = 0; i < n; ++i) {
+      KP kp = makeKP(b->ctx[i]);
+      kp.cur = v & 1;
+      kp.nxt = 1 - kp.cur;
+      kp.sweep = ((v >> 1) & 1) + 1;
+      kp.xdir = (v >> 2) & 1;
+      kp.fuseFinal = (v >> 3) & 1;
+      table[(size_t)v * n + i] = kp;
+    }
+  // virtual-block layout of every kernel
+  std::vector<unsigned> start((size_t)BK_COUNT * (n + 1)), gx((size_t)BK_COUNT * n);
+  for (int k = 0; k < BK_COUNT; ++k) {
+    unsigned acc = 0;
+    for (int i = 0; i < n; ++i) {
+      const mhy_ctx* x = b->ctx[i];
+      const long nx = x->o.nx, ny = x->o.ny;
+      dim3 g;
+      switch (k) {
+        case BK_DT:  // a few cells per thread: fewer, longer virtual blocks
+          g = dim3((unsigned)std::min<long>(kReduceBlocks, (x->ncell + 4 * kBlock - 1) / (4 * kBlock)));
+          break;
+        case BK_NODE: g = gridFor(x->nnode); break;
+        case BK_CELL: g = gridFor(x->ncell); break;
+        case BK_SWX: g = sweepWaveGrid(true, nx, ny); break;
+        case BK_SWY: g = sweepWaveGrid(false, nx, ny); break;
+        case BK_DUX: g = dualWaveGrid(true, nx, ny); break;
+        default: g = gridFor(x->nnode); break;  // per-node dual along y
+      }
+      start[(size_t)k * (n + 1) + i] = acc;
+      gx[(size_t)k * n + i] = g.x;
+      acc += g.x * g.y;
+    }
+    start[(size_t)k * (n + 1) + n] = acc;
+    b->total[k] = acc;
+  }
+  if (hipMalloc(&b->dTable, table.size() * sizeof(KP)) != hipSuccess ||
+      hipMalloc(&b->dStart, start.size() * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&b->dGx, gx.size() * sizeof(unsigned)) != hipSuccess)
+    return batchFail(b, MHY_ERR_DEVICE, "batch tables");
+  if (hipMemcpy(b->dTable, table.data(), table.size() * sizeof(KP), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->dStart, start.data(), start.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->dGx, gx.data(), gx.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess)
+    return batchFail(b, MHY_ERR_DEVICE, "batch tables upload");
+  if (!b->tune) b->tuneStep = kNCand;  // keep one workgroup per virtual block
+  return MHY_OK;
+}
+
+int mhy_batch_destroy(mhy_batch* b) {
+  if (!b) return MHY_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& e : b->exec)
+    if (e) (void)hipGraphExecDestroy(e);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->dTable) (void)hipFree(b->dTable);
+  if (b->dStart) (void)hipFree(b->dStart);
+  if (b->dGx) (void)hipFree(b->dGx);
+  for (mhy_ctx* x : b->ctx) mhy_destroy(x);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MHY_OK;
+}
+
+const char* mhy_batch_last_error(const mhy_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+long long mhy_batch_cells(const mhy_batch* b) { return b ? b->cells : 0; }
+
+int mhy_batch_context(mhy_batch* b, int i, mhy_ctx** ctx) {
+  if (!b || !ctx || i < 0 || i >= (int)b->ctx.size()) return MHY_ERR_ARG;
+  *ctx = b->ctx[i];
+  return MHY_OK;
+}
+
+int mhy_batch_run(mhy_batch* b, int nsteps) {
+  if (!b || nsteps < 0) return MHY_ERR_ARG;
+  if (b->ctx.empty() || !b->dTable) return MHY_ERR_ARG;
+  for (int s = 0; s < nsteps; ++s) {
+    int st;
+    if (b->tuneStep < kNCand) {
+      // one step with every kernel at candidate tuneStep, timed
+      for (int k = 0; k < BK_COUNT; ++k) b->blocks[k] = kCand[b->tuneStep];
+      if ((st = batchEnqueueStep(b, b->cur, b->xty, true)) != MHY_OK) return st;
+      if (hipStreamSynchronize(b->stream) != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, "batch sync");
+      const bool verbose = std::getenv("MAHYCO_BATCH_VERBOSE") != nullptr;
+      for (int k = 0; k < BK_COUNT; ++k) {  // every kernel runs once per step
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, b->ev[2 * k], b->ev[2 * k + 1]) == hipSuccess) b->tuneMs[k][b->tuneStep] = ms;
+        if (verbose) std::fprintf(stderr, "[batch tune] kernel %d blocks %u: %.4f ms\n", k, kCand[b->tuneStep], ms);
+      }
+      if (++b->tuneStep == kNCand) {
+        batchPickBlocks(b);
+        if (verbose)
+          for (int k = 0; k < BK_COUNT; ++k) std::fprintf(stderr, "[batch tune] kernel %d -> %u\n", k, b->blocks[k]);
+      }
+    } else if (b->useGraph) {
+      if (!b->exec[0] && (st = batchCapture(b)) != MHY_OK) return st;
+      if (hipGraphLaunch(b->exec[b->cur], b->stream) != hipSuccess)
+        return batchFail(b, MHY_ERR_DEVICE, "graph launch");
+    } else {
+      if ((st = batchEnqueueStep(b, b->cur, b->xty, false)) != MHY_OK) return st;
+    }
+    b->cur = 1 - b->cur;
+    b->xty = !b->xty;
+    ++b->iteration;
+  }
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, "batch sync");
+  // hand the state back to the problem contexts and collect their statuses
+  int status = MHY_OK;
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    mhy_ctx* x = b->ctx[i];
+    x->cur = b->cur;
+    x->x_then_y_n = b->xty;
+    x->iteration = (int)b->iteration;
+    Scalars sc;
+    if (hipMemcpy(&sc, x->f.sc, sizeof(Scalars), hipMemcpyDeviceToHost) != hipSuccess)
+      return batchFail(b, MHY_ERR_DEVICE, "batch scalars");
+    if (sc.err && status == MHY_OK) {
+      status = sc.err;
+      b->err = "problem " + std::to_string(i) + ": kernel status " + std::to_string(sc.err);
+      x->status = sc.err;
+    }
+  }
+  return status;
+}
+
+}  // extern "C"

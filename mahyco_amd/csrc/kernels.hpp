@@ -10,6 +10,15 @@ __global__ void k_dt_reduce(KP kp);
 __global__ void k_dt_hold(KP kp);
 __global__ void k_lagrange_node(KP kp);
 __global__ void k_lagrange_cell(KP kp);
+__global__ void k_dt_reduce_batch(BatchLaunch bl);
+__global__ void k_lagrange_node_batch(BatchLaunch bl);
+__global__ void k_lagrange_cell_batch(BatchLaunch bl);
+template <int M, bool PP, bool XD>
+__global__ void k_sweep_wave_batch(BatchLaunch bl);
+template <bool XD>
+__global__ void k_dual_wave_batch(BatchLaunch bl);
+template <bool XD>
+__global__ void k_dual_batch(BatchLaunch bl);
 template <int M, bool PP, bool XD>
 __global__ void k_sweep_cell(KP kp);
 template <int M, bool PP, bool XD>

@@ -46,14 +46,15 @@ __device__ __forceinline__ D2 nodeX(const KP& kp, long j, long i) {
 // max-pressure reduction of PhaseLagrange.cc:55-61.  Min and max ignore NaN
 // values like the reference's reducers; the last block to finish combines
 // the per-block partials.
-__global__ __launch_bounds__(256) void k_dt_reduce(KP kp) {
+// b: block index among the nb blocks working on this problem
+__device__ __forceinline__ void dtReduceBody(const KP& kp, unsigned b, unsigned nb) {
   const Opts& o = kp.o;
   const Fields& f = kp.f;
   const long nx = o.nx;
   double vmin = DBL_MAX, pm = -DBL_MAX;
   const double* ux = f.ux[kp.cur];
   const double* uy = f.uy[kp.cur];
-  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < kp.ncell; c += (long)gridDim.x * blockDim.x) {
+  for (long c = (long)b * blockDim.x + threadIdx.x; c < kp.ncell; c += (long)nb * blockDim.x) {
     const long j = c / nx, i = c - j * nx;
     const long p0 = j * (nx + 1) + i;
     const long nd[4] = {p0, p0 + 1, p0 + nx + 2, p0 + nx + 1};
@@ -75,38 +76,40 @@ __global__ __launch_bounds__(256) void k_dt_reduce(KP kp) {
   if (l == 0) smn[w] = vmin, smx[w] = pm;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double a = smn[0], b = smx[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) a = smin(a, smn[k]), b = (smx[k] > b) ? smx[k] : b;
-    f.partials[2 * blockIdx.x] = a;
-    f.partials[2 * blockIdx.x + 1] = b;
+    double lo = smn[0], hi = smx[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) lo = smin(lo, smn[k]), hi = (smx[k] > hi) ? smx[k] : hi;
+    f.partials[2 * b] = lo;
+    f.partials[2 * b + 1] = hi;
     __threadfence();
     unsigned int done = atomicAdd(&f.sc->done_blocks, 1u);
-    last = (done == gridDim.x - 1);
+    last = (done == nb - 1);
   }
   __syncthreads();
   if (!last) return;
-  double a = DBL_MAX, b = -DBL_MAX;
-  for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
-    a = smin(a, loadCoherent(&f.partials[2 * k]));
+  double lo = DBL_MAX, hi = -DBL_MAX;
+  for (int k = threadIdx.x; k < (int)nb; k += blockDim.x) {
+    lo = smin(lo, loadCoherent(&f.partials[2 * k]));
     double pb = loadCoherent(&f.partials[2 * k + 1]);
-    b = (pb > b) ? pb : b;
+    hi = (pb > hi) ? pb : hi;
   }
-  a = waveMin(a);
-  b = waveMax(b);
-  if (l == 0) smn[w] = a, smx[w] = b;
+  lo = waveMin(lo);
+  hi = waveMax(hi);
+  if (l == 0) smn[w] = lo, smx[w] = hi;
   __syncthreads();
   if (threadIdx.x == 0) {
-    a = smn[0], b = smx[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) a = smin(a, smn[k]), b = (smx[k] > b) ? smx[k] : b;
+    lo = smn[0], hi = smx[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) lo = smin(lo, smn[k]), hi = (smx[k] > hi) ? smx[k] : hi;
     Scalars* sc = f.sc;
     const double dtn = sc->dt[kp.cur];
-    const double dt1 = smin(a, 1.05 * dtn);
+    const double dt1 = smin(lo, 1.05 * dtn);
     sc->dt[kp.nxt] = dt1;
     sc->t[kp.nxt] = sc->t[kp.cur] + dt1;
-    sc->pmax = b;
+    sc->pmax = hi;
     sc->done_blocks = 0;
   }
 }
+
+__global__ __launch_bounds__(256) void k_dt_reduce(KP kp) { dtReduceBody(kp, blockIdx.x, gridDim.x); }
 
 // sansLagrange: deltat_nplus1 = deltat_n (VnrRemap.cc:143-146)
 __global__ void k_dt_hold(KP kp) {
@@ -122,10 +125,10 @@ __global__ void k_dt_hold(KP kp) {
 // updateVelocityBoundaryConditions (ConditionsLimites.cc:12-185) and
 // updatePosition (PhaseLagrange.cc:203-208).  In advection mode the velocity
 // is carried over unchanged (VnrRemap.cc:152-154).
-__global__ __launch_bounds__(256) void k_lagrange_node(KP kp) {
+__device__ __forceinline__ void lagrangeNodeBody(const KP& kp, unsigned b) {
   const Opts& o = kp.o;
   const Fields& f = kp.f;
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = (long)b * blockDim.x + threadIdx.x;
   if (p >= kp.nnode) return;
   const long nx = o.nx, ny = o.ny;
   const long j = p / (nx + 1), i = p - j * (nx + 1);
@@ -202,16 +205,18 @@ __global__ __launch_bounds__(256) void k_lagrange_node(KP kp) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_lagrange_node(KP kp) { lagrangeNodeBody(kp, blockIdx.x); }
+
 // Cell kernel: updateCellPos, computeSubVol, updateRho (+ periodic cells),
 // computeTau, computeDivU, computeArtificialViscosity, updateEnergy,
 // computeEOS, computePressionMoyenne (PhaseLagrange.cc:214-485,
 // ConditionsLimites.cc:186-209), then the cell part of the remap
 // preparation: computeVariablesForRemap and computeCellQuantitesForRemap
 // (PrepareRemap.cc:65-176).
-__global__ __launch_bounds__(256) void k_lagrange_cell(KP kp) {
+__device__ __forceinline__ void lagrangeCellBody(const KP& kp, unsigned b) {
   const Opts& o = kp.o;
   const Fields& f = kp.f;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = (long)b * blockDim.x + threadIdx.x;
   if (c >= kp.ncell) return;
   const int m = o.nbmat;
   const long nx = o.nx;
@@ -358,6 +363,26 @@ __global__ __launch_bounds__(256) void k_lagrange_cell(KP kp) {
   r = d2mul(r, 1.0 / (6.0 * V));
   f.XcL[c] = r.x;
   f.YcL[c] = r.y;
+}
+
+__global__ __launch_bounds__(256) void k_lagrange_cell(KP kp) { lagrangeCellBody(kp, blockIdx.x); }
+
+// ---------------------------------------------------------------- batches
+// One launch serves every problem of an ensemble: the virtual blocks of all
+// problems are numbered consecutively (BatchLaunch::start) and a fixed
+// number of workgroups walks them.
+
+__global__ __launch_bounds__(256) void k_dt_reduce_batch(BatchLaunch bl) {
+  batchForEach(bl, [&](const KP& kp, unsigned b, unsigned, unsigned nb) {
+    dtReduceBody(kp, b, nb);
+    __syncthreads();  // the shared scratch is reused by the next virtual block
+  });
+}
+__global__ __launch_bounds__(256) void k_lagrange_node_batch(BatchLaunch bl) {
+  batchForEach(bl, [&](const KP& kp, unsigned b, unsigned, unsigned) { lagrangeNodeBody(kp, b); });
+}
+__global__ __launch_bounds__(256) void k_lagrange_cell_batch(BatchLaunch bl) {
+  batchForEach(bl, [&](const KP& kp, unsigned b, unsigned, unsigned) { lagrangeCellBody(kp, b); });
 }
 
 }  // namespace mhy

@@ -64,10 +64,10 @@ __device__ __forceinline__ double limitedGradient(int lim, double gp, double gm,
 // remap/DualPhaseRemap2.cc:18-127 with remap/UtilesDualRemap.cc and
 // UtilesRemap-Impl.h:8-97).  XD: along x (left/right nodes).
 template <bool XD>
-__global__ __launch_bounds__(256) void k_dual(KP kp) {
+__device__ __forceinline__ void dualBody(const KP& kp, unsigned b) {
   const Opts& o = kp.o;
   const Fields& f = kp.f;
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = (long)b * blockDim.x + threadIdx.x;
   if (p >= kp.nnode) return;
   const int m = o.nbmat;
   const long nx = o.nx, ny = o.ny;
@@ -216,8 +216,18 @@ __global__ __launch_bounds__(256) void k_dual(KP kp) {
   }
 }
 
+template <bool XD>
+__global__ __launch_bounds__(256) void k_dual(KP kp) {
+  dualBody<XD>(kp, blockIdx.x);
+}
+template <bool XD>
+__global__ __launch_bounds__(256) void k_dual_batch(BatchLaunch bl) {
+  batchForEach(bl, [&](const KP& kp, unsigned local, unsigned, unsigned) { dualBody<XD>(kp, local); });
+}
 template __global__ void k_dual<true>(KP);
 template __global__ void k_dual<false>(KP);
+template __global__ void k_dual_batch<true>(BatchLaunch);
+template __global__ void k_dual_batch<false>(BatchLaunch);
 
 // The same nodal remap with one lane per node and the neighbour values taken
 // by wave shuffles.  A wave covers a strip of nodes along the sweep: along x,
@@ -235,8 +245,9 @@ constexpr int kDYOut = kDYRows - 6;
 __device__ __forceinline__ double dshfl(double v, int src) { return __shfl(v, src, kDW); }
 }  // namespace
 
+// b: linear block index (strips numbered row-major along x, as the sweeps)
 template <bool XD>
-__global__ __launch_bounds__(256) void k_dual_wave(KP kp) {
+__device__ __forceinline__ void dualWaveBody(const KP& kp, unsigned b) {
   constexpr int S = XD ? 1 : kDYCols;  // lane distance of the along-sweep neighbour
   const Opts& o = kp.o;
   const Fields& f = kp.f;
@@ -246,18 +257,21 @@ __global__ __launch_bounds__(256) void k_dual_wave(KP kp) {
   long a, t, na, nt;
   int slot, last;
   if (XD) {
-    const long seg = (long)blockIdx.x * 4 + wave;
+    const long nseg = (nx + 1 + kDXOut - 1) / kDXOut;  // strips per node row
+    const long g = (long)b * 4 + wave;
     slot = lane, last = kDW - 1;
-    a = seg * kDXOut - 3 + lane;
-    t = blockIdx.y;
+    a = (g % nseg) * kDXOut - 3 + lane;
+    t = g / nseg;
     na = nx, nt = ny;  // node along index in [0, na], across in [0, nt]
   } else {
+    const long gx = (nx + 1 + 4 * kDYCols - 1) / (4 * kDYCols);
+    const long bx = b % gx, by = b / gx;
     slot = lane / kDYCols, last = kDYRows - 1;
-    a = (long)blockIdx.y * kDYOut - 3 + slot;
-    t = (long)blockIdx.x * (4 * kDYCols) + wave * kDYCols + (lane % kDYCols);
+    a = by * kDYOut - 3 + slot;
+    t = bx * (4 * kDYCols) + wave * kDYCols + (lane % kDYCols);
     na = ny, nt = nx;
   }
-  const bool in = (a >= 0 && a <= na && t <= nt);
+  const bool in = (a >= 0 && a <= na && t >= 0 && t <= nt);
   const long j = XD ? t : a, i = XD ? a : t;
   const long p = j * (nx + 1) + i;
   const bool output = in && slot >= 3 && slot <= last - 3;
@@ -407,16 +421,27 @@ __global__ __launch_bounds__(256) void k_dual_wave(KP kp) {
   }
 }
 
+template <bool XD>
+__global__ __launch_bounds__(256) void k_dual_wave(KP kp) {
+  dualWaveBody<XD>(kp, blockIdx.x);
+}
+template <bool XD>
+__global__ __launch_bounds__(256) void k_dual_wave_batch(BatchLaunch bl) {
+  batchForEach(bl, [&](const KP& kp, unsigned local, unsigned, unsigned) { dualWaveBody<XD>(kp, local); });
+}
 template __global__ void k_dual_wave<true>(KP);
 template __global__ void k_dual_wave<false>(KP);
+template __global__ void k_dual_wave_batch<true>(BatchLaunch);
+template __global__ void k_dual_wave_batch<false>(BatchLaunch);
 
 // launch grid of the wave dual kernels (256-thread blocks of 4 waves)
 dim3 dualWaveGrid(bool xd, long nx, long ny) {
   if (xd) {
-    const long segs = (nx + 1 + kDXOut - 1) / kDXOut;
-    return dim3((unsigned)((segs + 3) / 4), (unsigned)(ny + 1));
+    const long strips = (nx + 1 + kDXOut - 1) / kDXOut * (ny + 1);
+    return dim3((unsigned)((strips + 3) / 4));
   }
-  return dim3((unsigned)((nx + 1 + 4 * kDYCols - 1) / (4 * kDYCols)), (unsigned)((ny + 1 + kDYOut - 1) / kDYOut));
+  const long gx = (nx + 1 + 4 * kDYCols - 1) / (4 * kDYCols), gy = (ny + 1 + kDYOut - 1) / kDYOut;
+  return dim3((unsigned)(gx * gy));
 }
 
 // Vnr::remapVariables, cells (PhaseRemapFinal.cc:23-205), then computeEOS

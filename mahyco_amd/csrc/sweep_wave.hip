@@ -53,8 +53,11 @@ __device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, kWav
 
 }  // namespace
 
+// b: linear block index.  Along x the strips of all rows are numbered
+// row-major and each wave of the block takes the next one; along y a block
+// covers 16 columns x 14 rows, blocks ordered column-group-major per row band.
 template <int M, bool PP, bool XD>
-__global__ __launch_bounds__(256) void k_sweep_wave(KP kp) {
+__device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
   constexpr int NC = 3 * M + 1;
   constexpr int S = XD ? 1 : kYCols;  // lane distance of the along-sweep neighbour
   const Opts& o = kp.o;
@@ -66,18 +69,22 @@ __global__ __launch_bounds__(256) void k_sweep_wave(KP kp) {
   long a, t, na, nt;
   int slot;  // position along the sweep inside the wave strip (0 and last: halo)
   if (XD) {
-    const long seg = (long)blockIdx.x * 4 + wave;
+    const long nseg = (nx + kXOut - 1) / kXOut;  // strips per row
+    const long g = (long)b * 4 + wave;
+    const long seg = g % nseg;
     slot = lane;
     a = seg * kXOut - 1 + lane;
-    t = blockIdx.y;
+    t = g / nseg;  // rows beyond ny have no cells
     na = nx, nt = ny;
   } else {
+    const long gx = (nx + 4 * kYCols - 1) / (4 * kYCols);
+    const long bx = b % gx, by = b / gx;
     slot = lane / kYCols;
-    a = (long)blockIdx.y * kYOut - 1 + slot;
-    t = (long)blockIdx.x * (4 * kYCols) + wave * kYCols + (lane % kYCols);
+    a = by * kYOut - 1 + slot;
+    t = bx * (4 * kYCols) + wave * kYCols + (lane % kYCols);
     na = ny, nt = nx;
   }
-  const bool inA = (a >= 0 && a < na), inT = (t < nt);
+  const bool inA = (a >= 0 && a < na), inT = (t >= 0 && t < nt);
   const bool in = inA && inT;
   const long j = XD ? t : a, i = XD ? a : t;
   const long c = j * nx + i;
@@ -262,9 +269,22 @@ __global__ __launch_bounds__(256) void k_sweep_wave(KP kp) {
   }
 }
 
-#define MHY_SWEEP_WAVE_INST(M, PP)                       \
-  template __global__ void k_sweep_wave<M, PP, true>(KP); \
-  template __global__ void k_sweep_wave<M, PP, false>(KP);
+template <int M, bool PP, bool XD>
+__global__ __launch_bounds__(256) void k_sweep_wave(KP kp) {
+  sweepWaveBody<M, PP, XD>(kp, blockIdx.x);
+}
+
+// all problems of an ensemble in one launch (BatchLaunch, common.hpp)
+template <int M, bool PP, bool XD>
+__global__ __launch_bounds__(256) void k_sweep_wave_batch(BatchLaunch bl) {
+  batchForEach(bl, [&](const KP& kp, unsigned local, unsigned, unsigned) { sweepWaveBody<M, PP, XD>(kp, local); });
+}
+
+#define MHY_SWEEP_WAVE_INST(M, PP)                                      \
+  template __global__ void k_sweep_wave<M, PP, true>(KP);                \
+  template __global__ void k_sweep_wave<M, PP, false>(KP);               \
+  template __global__ void k_sweep_wave_batch<M, PP, true>(BatchLaunch); \
+  template __global__ void k_sweep_wave_batch<M, PP, false>(BatchLaunch);
 MHY_SWEEP_WAVE_INST(1, false)
 MHY_SWEEP_WAVE_INST(1, true)
 MHY_SWEEP_WAVE_INST(2, false)
@@ -276,10 +296,11 @@ MHY_SWEEP_WAVE_INST(3, true)
 // launch grid of the wave sweeps (256-thread blocks of 4 waves)
 dim3 sweepWaveGrid(bool xd, long nx, long ny) {
   if (xd) {
-    const long segs = (nx + kXOut - 1) / kXOut;
-    return dim3((unsigned)((segs + 3) / 4), (unsigned)ny);
+    const long strips = (nx + kXOut - 1) / kXOut * ny;
+    return dim3((unsigned)((strips + 3) / 4));
   }
-  return dim3((unsigned)((nx + 4 * kYCols - 1) / (4 * kYCols)), (unsigned)((ny + kYOut - 1) / kYOut));
+  const long gx = (nx + 4 * kYCols - 1) / (4 * kYCols), gy = (ny + kYOut - 1) / kYOut;
+  return dim3((unsigned)(gx * gy));
 }
 
 }  // namespace mhy

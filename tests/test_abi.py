@@ -88,3 +88,12 @@ def test_reader_positional_quirk():
 def test_missing_file_is_an_io_error(lib):
     cfg = abi.MhyConfig()
     assert lib.mhy_read_donnees(b"/nonexistent/donnees.txt", ctypes.byref(cfg)) == abi.MHY_ERR_IO
+
+
+def test_batch_rejects_mixed_material_counts(lib):
+    """Argument checks of mhy_batch_create run before any device call."""
+    from mahyco_amd import configs
+    a = configs.riemann(16, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
+    b = configs.triple_layout(30, 10)
+    with pytest.raises(mahyco_amd.MahycoError, match="material count"):
+        mahyco_amd.Batch([a, b])
