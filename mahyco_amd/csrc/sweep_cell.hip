@@ -98,11 +98,11 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
     const int nload = PP ? 3 * M : NC;
 #pragma unroll
     for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cd] : 0.0;
-    cellPhi<M, PP>(u, PP ? 1.0 : f.VL[cd], PP ? f.qL[cd] : 0.0, s[d].phi);
+    cellPhi<M, PP, double>(u, PP ? 1.0 : f.VL[cd], PP ? f.qL[cd] : 0.0, s[d].phi);
     s[d].xc = XD ? f.XcL[cd] : f.YcL[cd];
     s[d].H = needH ? widthC<XD>(kp, dt1, XD ? j : b, XD ? b : i) : 0.0;
     s[d].fl = 0;
-    if (mixteOpt) s[d].fl = (kp.sweep == 1) ? packFlags(f.mixte[cd], f.pure[cd]) : packedFlags<M>(s[d].phi);
+    if (mixteOpt) s[d].fl = (kp.sweep == 1) ? packFlags(f.mixte[cd], f.pure[cd]) : packedFlags<M, double>(s[d].phi);
   }
   // a missing neighbour takes the values of the cell it neighbours
   // (PhaseRemap1.cc:157-158)
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
 
   // neighbourhood of slot d (1..3)
   auto nbr = [&](const Slot<NC>& sm, const Slot<NC>& s0, const Slot<NC>& sp, bool inM, bool inP) {
-    Nbr n{sm.phi, s0.phi, sp.phi, sm.xc, s0.xc, sp.xc, inM, inP, s0.H, sm.H, sp.H, false};
+    Nbr<double> n{sm.phi, s0.phi, sp.phi, sm.xc, s0.xc, sp.xc, inM, inP, s0.H, sm.H, sp.H, false};
     if (mixteOpt) n.vp = pureNbh(sm.fl, s0.fl, sp.fl);
     return n;
   };
@@ -154,18 +154,18 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
       selectSlot(u0, upLow, sl1, sl2);
       selectSlot(up, upLow, sl2, sl3);
       const bool inM = upLow ? in0 : in1, inP = upLow ? in2 : in3;
-      const Nbr n = nbr(um, u0, up, inM, inP);
+      const Nbr<double> n = nbr(um, u0, up, inM, inP);
       const double xf = XD ? 0.5 * (o.dx * (double)b + o.dx * (double)b) : 0.5 * (o.dy * (double)b + o.dy * (double)b);
       const double sx = 0.0 + (xf - u0.xc);
 #pragma unroll
-      for (int q = 0; q < NC; ++q) out[q] = u0.phi[q] + gradQ<XD>(o, n, q) * sx;
+      for (int q = 0; q < NC; ++q) out[q] = u0.phi[q] + gradQ<XD, double>(o, n, q) * sx;
     };
     if (hasM) faceValue(a, s[0], s[1], s[2], s[3], in[0], in[1], in[2], in[3], vm, phiM);
     if (hasP) faceValue(a + 1, s[1], s[2], s[3], s[4], in[1], in[2], in[3], in[4], vp, phiP);
   } else if (o.order > 1) {
     // PenteBorne: phi_f = Av(minus cell) - Ar(plus cell) (PhaseRemap1.cc:361-364)
     double Ar0[NC], Av0[NC], Dual[M];
-    ppFluxes<M, XD>(o, nbr(s[1], s[2], s[3], in[1], in[3]), vm, vp, dtn, pb, Ar0, Av0, Dual);
+    ppFluxes<M, XD, double>(o, nbr(s[1], s[2], s[3], in[1], in[3]), vm, vp, dtn, pb, Ar0, Av0, Dual);
     if (pb) {
       const double w = (Lm + Lp) * 0.5;
 #pragma unroll
@@ -173,13 +173,13 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
     }
     if (hasM) {
       double ArM[NC], AvM[NC], DM[M];
-      ppFluxes<M, XD, false, true>(o, nbr(s[0], s[1], s[2], in[0], in[2]), fnv(a - 1), vm, dtn, false, ArM, AvM, DM);
+      ppFluxes<M, XD, double, false, true>(o, nbr(s[0], s[1], s[2], in[0], in[2]), fnv(a - 1), vm, dtn, false, ArM, AvM, DM);
 #pragma unroll
       for (int q = 0; q < NC; ++q) phiM[q] = AvM[q] - Ar0[q];
     }
     if (hasP) {
       double ArP[NC], AvP[NC], DP[M];
-      ppFluxes<M, XD, true, false>(o, nbr(s[2], s[3], s[4], in[2], in[4]), vp, fnv(a + 2), dtn, false, ArP, AvP, DP);
+      ppFluxes<M, XD, double, true, false>(o, nbr(s[2], s[3], s[4], in[2], in[4]), vp, fnv(a + 2), dtn, false, ArP, AvP, DP);
 #pragma unroll
       for (int q = 0; q < NC; ++q) phiP[q] = Av0[q] - ArP[q];
     }

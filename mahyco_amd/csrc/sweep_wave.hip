@@ -106,11 +106,11 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
     const int nload = PP ? 3 * M : NC;
 #pragma unroll
     for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cb] : 0.0;
-    cellPhi<M, PP>(u, PP ? 1.0 : f.VL[cb], PP ? f.qL[cb] : 0.0, ph);
+    cellPhi<M, PP, double>(u, PP ? 1.0 : f.VL[cb], PP ? f.qL[cb] : 0.0, ph);
     x = XD ? f.XcL[cb] : f.YcL[cb];
     h = needH ? widthW<XD>(kp, dt1, jb, ib) : 0.0;
     flg = 0;
-    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<M>(ph);
+    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<M, double>(ph);
   };
   double phi[NC], xc = 0.0, H = 0.0;
   int fl = 0;
@@ -138,7 +138,7 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
   fm = inM ? fm : fl, fp = inP ? fp : fl;
   if (in && slot == 0 && inM) loadCell(a - 1, phm, xcm, Hm, fm);
   if (in && slot == lastSlot && inP) loadCell(a + 1, php, xcp, Hp, fp);
-  Nbr n{phm, phi, php, xcm, xc, xcp, inM, inP, H, Hm, Hp, false};
+  Nbr<double> n{phm, phi, php, xcm, xc, xcp, inM, inP, H, Hm, Hp, false};
   if (mixteOpt) n.vp = ((fl >> 8) == 0) && fl == fp && fl == fm;
 
   // normal velocities of the minus / plus faces (PrepareRemap.cc:50-62)
@@ -169,7 +169,7 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
     // y: above/below
     double g[NC], gm[NC], gp[NC];
 #pragma unroll
-    for (int q = 0; q < NC; ++q) g[q] = in ? gradQ<XD>(o, n, q) : 0.0;
+    for (int q = 0; q < NC; ++q) g[q] = in ? gradQ<XD, double>(o, n, q) : 0.0;
 #pragma unroll
     for (int q = 0; q < NC; ++q) gm[q] = shfl(g[q], lm), gp[q] = shfl(g[q], lp);
     // face between along-cells lo (lower index) and hi
@@ -192,7 +192,7 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
 #pragma unroll
     for (int k = 0; k < M; ++k) Dual[k] = 0.0;
     const bool formed = o.order > 1;  // the fluxes are only formed for order > 1
-    if (formed && in) ppFluxes<M, XD>(o, n, vm, vp, dtn, pb && output, Ar, Av, Dual);
+    if (formed && in) ppFluxes<M, XD, double>(o, n, vm, vp, dtn, pb && output, Ar, Av, Dual);
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
       const double avm = shfl(Av[q], lm), arp = shfl(Ar[q], lp);
