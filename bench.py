@@ -6,7 +6,10 @@ over the whole mesh; the metric is cell-updates/s = cells x steps / wall
 time of the timed steps, with the state resident in HBM.
 
 Default workload: the two-material Riemann problem on 2048^2 with a
-perturbed interface, superBee + PenteBorne (configuration C2).
+perturbed interface, superBee + PenteBorne, FP64.  --config c4 runs the
+three-material layout in the mixed FP32 / FP64 form with a per-field error
+report against the FP64 form after the run; --config c5 runs the 512-problem
+ensemble batched (mixed form by default) against the plain FP64 form.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -23,29 +26,40 @@ METRIC = "cell-updates/s (Lagrange+remap) + HBM GB/s fraction; mixed/batched vs 
 HBM_PEAK = 8.0e12
 
 
+PRECISION_NAME = {0: "FP64", 1: "mixed FP32/FP64"}
+DTYPE = {0: "f64", 1: "f32+f64"}
+
+
 def workload(name):
     from mahyco_amd import configs
     if name == "c2":
-        return "two-material Riemann 2048x2048, perturbed interface, superBee+PenteBorne, FP64", configs.riemann_perturbed()
+        return "two-material Riemann 2048x2048, perturbed interface, superBee+PenteBorne", configs.riemann_perturbed()
     if name == "c3":
-        return "point explosion 4096x4096, 1 material, minmod order 2, FP64", configs.point_explosion()
+        return "point explosion 4096x4096, 1 material, minmod order 2", configs.point_explosion()
     if name == "c4":
-        return "three-material layout 3072x1024, minmod, FP64", configs.triple_layout()
+        return "three-material layout 3072x1024, minmod, density jitter 1e-3", configs.triple_layout()
     if name.startswith("riemann@"):
         n = int(name[8:])
         c = configs.riemann(n, 0.45, 1.0, 1.0, 0.125, 0.1, 1.4, 1.67)
-        return f"two-material Riemann {n}x{n}, straight interface, minmod, FP64", c
+        return f"two-material Riemann {n}x{n}, straight interface, minmod", c
     if name.startswith("c2@"):
         n = int(name[3:])
         c = configs.riemann_perturbed(n=n)
-        return f"two-material Riemann {n}x{n}, perturbed interface, superBee+PenteBorne, FP64", c
+        return f"two-material Riemann {n}x{n}, perturbed interface, superBee+PenteBorne", c
+    if name.startswith("c4@"):
+        nx = int(name[3:])
+        return f"three-material layout {nx}x{nx // 3}, minmod, density jitter 1e-3", configs.triple_layout(nx, nx // 3)
     raise SystemExit(f"unknown config {name}")
 
 
-def algorithmic_bytes_per_cell(m):
-    # compulsory persistent state per cell-update, read once and written once:
-    # cell rho,p,q,c,divu + 7 words per material, mixture e written, node
-    # u_x,u_y,M, mixed/pure flags (4 B read + 4 B written each)
+def algorithmic_bytes_per_cell(m, precision=0):
+    """Compulsory persistent state per cell-update, read once and written
+    once: cell rho, p, q, c, divu + 7 words per material (f_k, mu_k, rho_k,
+    e_k, p_k, q_k, M_k), mixture e written, node u_x, u_y, M, and the
+    mixed/pure flags (4 B read + 4 B written each).  FP64: 8 B words; mixed
+    form: f_k and M_k 8 B, the rest 4 B."""
+    if precision == 1:
+        return 4 * (17 + 10 * m) + 32 * m + 16
     return 8 * (17 + 14 * m) + 16
 
 
@@ -72,6 +86,35 @@ def cpu_baseline(cfg, seconds_budget):
                       f"mesh, {n} timed steps after 1 warm-up step"}
 
 
+def precision_report(cfg, steps):
+    """Per-field relative L1 / Linf errors of the mixed form against the FP64
+    form after `steps` steps from the same initial state, the two forms'
+    cell-update rates over those steps, and the total-mass drift of each."""
+    import mahyco_amd
+    from mahyco_amd import precision
+    rates, ctxs, mass0 = {}, {}, None
+    for prec in (0, 1):
+        cfg.precision = prec
+        ctx = mahyco_amd.Context(cfg)
+        if mass0 is None:
+            mass0 = ctx.conservation()[0]
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(steps)
+        rates[prec] = cfg.nx * cfg.ny * steps / (time.perf_counter() - t0)
+        ctxs[prec] = ctx
+    errs = precision.field_errors(ctxs[1], ctxs[0], cfg.nbmat)
+    drift = {DTYPE[p]: abs(ctxs[p].conservation()[0] - mass0) / mass0 for p in (0, 1)}
+    t = ctxs[0].time_state().t_n
+    for c in ctxs.values():
+        c.close()
+    cfg.precision = 1
+    return {"steps": steps, "t_end": t, "against": "FP64 form (bitwise equal to the CPU oracle)",
+            "fields": {k: {"rel_L1": v[0], "rel_Linf": v[1]} for k, v in errs.items()},
+            "mass_drift": drift, "fp64_rate": rates[0], "mixed_rate": rates[1],
+            "mixed_over_fp64": rates[1] / rates[0]}
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of known size (the runtime's copy
     kernel), used to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
@@ -94,11 +137,15 @@ def copy_probe(mib):
 
 def bench_ensemble(args):
     """C5: 512 two-material Riemann problems of uneven sizes stepped together
-    (one launch per kernel over all problems, HIP graphs), against the same
-    problems run one after another with the single-problem path."""
+    (one launch per kernel over all problems, HIP graphs), in the mixed form
+    unless --precision fp64, against the same problems run one after another
+    with the plain single-problem FP64 path."""
     import mahyco_amd
     from mahyco_amd import configs
+    prec = 1 if args.precision in (None, "mixed") else 0
     cfgs = configs.ensemble(n=args.ensemble)
+    for c in cfgs:
+        c.precision = prec
     batch = mahyco_amd.Batch(cfgs)
     cells = batch.cells
     batch.run(max(args.warmup, 6))  # the first steps also tune the launch widths
@@ -107,10 +154,12 @@ def bench_ensemble(args):
     wall = time.perf_counter() - t0
     batch.close()
     value = cells * args.steps / wall
-    # the same problems, one at a time (bounded number of steps)
+    # the same problems in the plain FP64 form, one at a time (bounded number
+    # of steps)
     plain_steps = max(1, min(args.steps, args.plain_steps))
     plain_t = 0.0
     for c in cfgs:
+        c.precision = 0
         ctx = mahyco_amd.Context(c)
         ctx.run(1)
         t1 = time.perf_counter()
@@ -118,18 +167,18 @@ def bench_ensemble(args):
         plain_t += time.perf_counter() - t1
         ctx.close()
     plain = cells * plain_steps / plain_t
-    B = algorithmic_bytes_per_cell(2)
+    B = algorithmic_bytes_per_cell(2, prec)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": DTYPE[prec], "data": "synthetic",
         "config": {"workload": f"ensemble of {len(cfgs)} two-material Riemann problems, sides 64..1024 "
-                               f"log-uniform (seed 20261204), minmod, FP64", "problems": len(cfgs), "cells": cells,
-                   "parallelism": "single GPU"},
+                               f"log-uniform (seed 20261204), minmod, batched, {PRECISION_NAME[prec]}",
+                   "problems": len(cfgs), "cells": cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "whole step (batched)", "achieved": B * value / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B * value / HBM_PEAK, "traffic": None},
         "plain_form": {"value": plain, "unit": "cell-updates/s", "steps": plain_steps,
-                       "note": "same problems, one context after another (mhy_run per problem)"},
+                       "note": "same problems, FP64, one context after another (mhy_run per problem)"},
         "batched_over_plain": value / plain,
     }
     print(json.dumps(out), flush=True)
@@ -146,6 +195,10 @@ def main():
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--ensemble", type=int, default=512, help="problems of the c5 ensemble")
     ap.add_argument("--plain-steps", type=int, default=5, help="steps of the one-by-one comparison (c5)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="fp64 or mixed FP32/FP64 (default: mixed for c4 and c5, fp64 otherwise)")
+    ap.add_argument("--report-steps", type=int, default=500,
+                    help="steps of the per-field error report against FP64 (mixed form)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -161,6 +214,11 @@ def main():
     if args.config == "c5":
         return bench_ensemble(args)
     desc, cfg = workload(args.config)
+    if args.precision is None:
+        args.precision = "mixed" if args.config.startswith("c4") else "fp64"
+    prec = 1 if args.precision == "mixed" else 0
+    cfg.precision = prec
+    desc = f"{desc}, {PRECISION_NAME[prec]}"
     ctx = mahyco_amd.Context(cfg)
     cells = cfg.nx * cfg.ny
     ctx.run(args.warmup)
@@ -187,7 +245,7 @@ def main():
                 traffic = json.load(fh).get(args.config, {}).get(dom)
         except (OSError, ValueError):
             traffic = None
-    B = algorithmic_bytes_per_cell(cfg.nbmat)
+    B = algorithmic_bytes_per_cell(cfg.nbmat, prec)
     out = {
         "metric": METRIC,
         "value": value,
@@ -199,7 +257,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": DTYPE[prec],
         "data": "synthetic",
         "config": {"workload": desc, "mesh": [cfg.nx, cfg.ny], "nbmat": cfg.nbmat, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
@@ -209,7 +267,10 @@ def main():
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "GBps_alg": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)
                         if v["avg_ms"] > 0 else 0} for k, v in stats.items() if v["launches"]},
     }
+    if prec == 1 and args.report_steps > 0:
+        out["precision_report"] = precision_report(cfg, args.report_steps)
     if not args.no_cpu_baseline:
+        cfg.precision = 0
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

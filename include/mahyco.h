@@ -129,7 +129,11 @@ typedef struct mhy_config {
   int sans_lagrange;
   /* extension keys */
   int abort_on_energy;      /* 1: stop with MHY_ERR_ENERGY when |e_k|>5 */
-  int precision;            /* 0: FP64 everywhere                      */
+  int precision;            /* 0: FP64 everywhere; 1: mixed form: FP32
+                               storage of the bulk cell / node fields and
+                               FP32 slope / limiter / PenteBorne arithmetic,
+                               FP64 volume fractions, masses, remap vectors,
+                               compensated flux accumulation, dt          */
   unsigned long long seed;  /* initialiser randomisation               */
   /* initialiser parameters:
    *  RIEMANN_PERTURBED: [0] mean interface x, [1] amplitude, [2] modes,

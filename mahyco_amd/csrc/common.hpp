@@ -7,6 +7,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <cstring>
 
 #include "../../include/mahyco.h"
 
@@ -95,19 +96,26 @@ struct Opts {
   int flux_left, flux_right, flux_bottom, flux_top, flux_any;
 };
 
-// Device pointers of one problem (structure of arrays, FP64, cell index
-// j*nx+i, node index j*(nx+1)+i).  [2] arrays are the reference's n/n+1
-// pairs, addressed through the parity of the launch.
-struct Fields {
-  double *rho[2], *p[2], *q[2], *c[2], *divu[2], *e[2];
-  double *rho_k[2][NMAT], *p_k[2][NMAT], *q_k[2][NMAT], *c_k[2][NMAT], *e_k[2][NMAT];
-  double *ux[2], *uy[2];
+// Device pointers of one problem (structure of arrays, cell index j*nx+i,
+// node index j*(nx+1)+i).  [2] arrays are the reference's n/n+1 pairs,
+// addressed through the parity of the launch.  R is the storage type of the
+// bulk cell / node fields: double for the FP64 form, float for the mixed
+// form.  Volume fractions, masses, the remap vectors and fluxes, the nodal
+// momentum of the dual remap, coordinates and the time-step scalars are FP64
+// in both forms.
+template <class R>
+struct FieldsT {
+  R *rho[2], *p[2], *q[2], *c[2], *divu[2], *e[2];
+  R *rho_k[2][NMAT], *p_k[2][NMAT], *q_k[2][NMAT], *c_k[2][NMAT], *e_k[2][NMAT];
+  R *ux[2], *uy[2];
   double *X[2], *Y[2];  // moving node coordinates (MODE_LAGRANGE)
   double *vol[2];       // sum of the corner sub-volumes of the step start
-  double *f_k[NMAT], *mu_k[NMAT], *M, *M_k[NMAT];
+  double *f_k[NMAT];
+  R *mu_k[NMAT];
+  double *M, *M_k[NMAT];
   int *mixte, *pure;
-  double *Mp, *outvx, *outvy;
-  double *rho0, *e0, *rho_k0[NMAT], *e_k0[NMAT];  // t=0 fields (periodic cells)
+  R *Mp, *outvx, *outvy;
+  R *rho0, *e0, *rho_k0[NMAT], *e_k0[NMAT];  // t=0 fields (periodic cells)
   // per-step intermediates
   double *UL[NCMAX], *U1[NCMAX], *U2[NCMAX];
   double *VL, *XcL, *YcL, *rL, *qL;
@@ -120,8 +128,9 @@ struct Fields {
 };
 
 // Parameters of one launch: fields, options, parity and sweep direction.
-struct KP {
-  Fields f;
+template <class R>
+struct KPT {
+  FieldsT<R> f;
   Opts o;
   int cur, nxt;  // parity of the n and n+1 buffers
   int sweep;     // 1 or 2
@@ -130,19 +139,40 @@ struct KP {
   long ncell, nnode;
 };
 
+using Fields = FieldsT<double>;
+using KP = KPT<double>;
+
+// true for the mixed form (FP32 bulk storage)
+template <class R>
+constexpr bool kMixed = sizeof(R) == 4;
+
+// The same parameter block seen with another bulk storage type: every field
+// member is a pointer, so the layouts are identical and the host keeps one
+// untyped copy (KP) whatever the form.
+template <class R>
+KPT<R> retype(const KP& kp) {
+  static_assert(sizeof(KPT<R>) == sizeof(KP), "parameter blocks differ");
+  KPT<R> r;
+  memcpy(&r, &kp, sizeof(r));
+  return r;
+}
+
 // One launch over an ensemble of problems: the launch parameters of problem
 // p for step variant v are table[v * nprob + p]; the virtual blocks of problem
 // p are start[p] .. start[p+1]-1, laid out as rows of gx[p] blocks.
-struct BatchLaunch {
-  const KP* table;
+template <class R>
+struct BatchLaunchT {
+  const KPT<R>* table;
   const unsigned* start;  // nprob + 1 entries
   const unsigned* gx;     // nprob entries
   int nprob, variant;
   unsigned total;         // start[nprob]
 };
+using BatchLaunch = BatchLaunchT<double>;
 
 // problem p and block index inside it of virtual block vb (binary search)
-__device__ __forceinline__ void batchLocate(const BatchLaunch& bl, unsigned vb, int& p, unsigned& local) {
+template <class R>
+__device__ __forceinline__ void batchLocate(const BatchLaunchT<R>& bl, unsigned vb, int& p, unsigned& local) {
   int lo = 0, hi = bl.nprob - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -156,8 +186,8 @@ __device__ __forceinline__ void batchLocate(const BatchLaunch& bl, unsigned vb, 
 // Runs fn(kp, local block, gx, blocks of the problem) for a contiguous range
 // of virtual blocks: workgroup w takes blocks [w*per, (w+1)*per), so that it
 // stays on one problem (and its launch parameters) for as long as possible.
-template <class F>
-__device__ __forceinline__ void batchForEach(const BatchLaunch& bl, F&& fn) {
+template <class R, class F>
+__device__ __forceinline__ void batchForEach(const BatchLaunchT<R>& bl, F&& fn) {
   const unsigned per = (bl.total + gridDim.x - 1) / gridDim.x;
   unsigned vb = blockIdx.x * per;
   const unsigned end = (vb + per < bl.total) ? vb + per : bl.total;

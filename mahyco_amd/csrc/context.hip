@@ -33,7 +33,10 @@ constexpr int kReduceBlocks = 1024;
 struct mhy_ctx {
   mhy_config cfg;
   Opts o;
+  // device pointers; in the mixed form the bulk fields (FieldsT<float>
+  // members of type R) point to FP32 arrays and are retyped at launch
   Fields f;
+  bool mixed = false;  // precision 1: FP32 bulk storage and sweep arithmetic
   HostInit init;
   hipStream_t stream = nullptr;
   int device = 0;
@@ -90,6 +93,20 @@ double* allocD(mhy_ctx* x, long n, hipError_t& e) {
   if (e == hipSuccess) {
     x->allocs.push_back(p);
     e = hipMemset(p, 0, sizeof(double) * (size_t)std::max(n, 1L));
+  }
+  return static_cast<double*>(p);
+}
+
+// bulk-field array: FP64, or FP32 in the mixed form (kept as an untyped
+// pointer in Fields)
+double* allocR(mhy_ctx* x, long n, hipError_t& e) {
+  if (e != hipSuccess) return nullptr;
+  void* p = nullptr;
+  const size_t bytes = (x->mixed ? sizeof(float) : sizeof(double)) * (size_t)std::max(n, 1L);
+  e = hipMalloc(&p, bytes);
+  if (e == hipSuccess) {
+    x->allocs.push_back(p);
+    e = hipMemset(p, 0, bytes);
   }
   return static_cast<double*>(p);
 }
@@ -162,6 +179,14 @@ void collectTimes(mhy_ctx* x) {
 
 dim3 gridFor(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
+// launches kd (FP64 form) or kf (mixed form) on the context stream with the
+// parameter block seen with the matching storage type
+template <class KD, class KF, class... A>
+void launchR(const mhy_ctx* x, KD kd, KF kf, dim3 grid, dim3 block, const KP& kp, A... args) {
+  if (x->mixed) hipLaunchKernelGGL(kf, grid, block, 0, x->stream, retype<float>(kp), args...);
+  else hipLaunchKernelGGL(kd, grid, block, 0, x->stream, kp, args...);
+}
+
 int checkLaunch(mhy_ctx* x, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(x, MHY_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
@@ -171,32 +196,44 @@ int checkLaunch(mhy_ctx* x, const char* what) {
 int phaseDt(mhy_ctx* x) {
   KP kp = makeKP(x);
   if (x->o.mode == MODE_ADVECTION) {
-    timed(x, K_DT, [&] { hipLaunchKernelGGL(k_dt_hold, dim3(1), dim3(64), 0, x->stream, kp); });
+    timed(x, K_DT, [&] { launchR(x, k_dt_hold<double>, k_dt_hold<float>, dim3(1), dim3(64), kp); });
   } else {
     const unsigned nb = (unsigned)std::min<long>(kReduceBlocks, (x->ncell + kBlock - 1) / kBlock);
-    timed(x, K_DT, [&] { hipLaunchKernelGGL(k_dt_reduce, dim3(nb), dim3(kBlock), 0, x->stream, kp); });
+    timed(x, K_DT, [&] { launchR(x, k_dt_reduce<double>, k_dt_reduce<float>, dim3(nb), dim3(kBlock), kp); });
   }
   return checkLaunch(x, "dt");
 }
 
 int phaseLagrange(mhy_ctx* x) {
   KP kp = makeKP(x);
-  timed(x, K_NODE, [&] { hipLaunchKernelGGL(k_lagrange_node, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
-  timed(x, K_CELL, [&] { hipLaunchKernelGGL(k_lagrange_cell, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+  timed(x, K_NODE, [&] {
+    launchR(x, k_lagrange_node<double>, k_lagrange_node<float>, gridFor(x->nnode), dim3(kBlock), kp);
+  });
+  timed(x, K_CELL, [&] {
+    launchR(x, k_lagrange_cell<double>, k_lagrange_cell<float>, gridFor(x->ncell), dim3(kBlock), kp);
+  });
   return checkLaunch(x, "lagrange");
 }
 
 // sweep kernels for the direction, the material count and the PenteBorne
 // option: the wave-shuffle kernels (default) or the per-cell stencil kernels
+// (FP64 form only)
 using KernelFn = void (*)(KP);
+using KernelFnF = void (*)(KPT<float>);
+template <class R>
+void (*sweepWaveKernel(bool xd, int m, bool pp))(KPT<R>) {
+  using Fn = void (*)(KPT<R>);
+  static const Fn kw[2][3][2] = {
+      {{k_sweep_wave<R, 1, false, false>, k_sweep_wave<R, 1, true, false>},
+       {k_sweep_wave<R, 2, false, false>, k_sweep_wave<R, 2, true, false>},
+       {k_sweep_wave<R, 3, false, false>, k_sweep_wave<R, 3, true, false>}},
+      {{k_sweep_wave<R, 1, false, true>, k_sweep_wave<R, 1, true, true>},
+       {k_sweep_wave<R, 2, false, true>, k_sweep_wave<R, 2, true, true>},
+       {k_sweep_wave<R, 3, false, true>, k_sweep_wave<R, 3, true, true>}}};
+  return kw[xd ? 1 : 0][m - 1][pp ? 1 : 0];
+}
 KernelFn sweepKernel(bool wave, bool xd, int m, bool pp) {
-  static const KernelFn kw[2][3][2] = {
-      {{k_sweep_wave<1, false, false>, k_sweep_wave<1, true, false>},
-       {k_sweep_wave<2, false, false>, k_sweep_wave<2, true, false>},
-       {k_sweep_wave<3, false, false>, k_sweep_wave<3, true, false>}},
-      {{k_sweep_wave<1, false, true>, k_sweep_wave<1, true, true>},
-       {k_sweep_wave<2, false, true>, k_sweep_wave<2, true, true>},
-       {k_sweep_wave<3, false, true>, k_sweep_wave<3, true, true>}}};
+  if (wave) return sweepWaveKernel<double>(xd, m, pp);
   static const KernelFn kc[2][3][2] = {
       {{k_sweep_cell<1, false, false>, k_sweep_cell<1, true, false>},
        {k_sweep_cell<2, false, false>, k_sweep_cell<2, true, false>},
@@ -204,7 +241,7 @@ KernelFn sweepKernel(bool wave, bool xd, int m, bool pp) {
       {{k_sweep_cell<1, false, true>, k_sweep_cell<1, true, true>},
        {k_sweep_cell<2, false, true>, k_sweep_cell<2, true, true>},
        {k_sweep_cell<3, false, true>, k_sweep_cell<3, true, true>}}};
-  return (wave ? kw : kc)[xd ? 1 : 0][m - 1][pp ? 1 : 0];
+  return kc[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
 
 // One remap sweep: stage 1 = cell fluxes and update (computeGradPhiFace ..
@@ -220,19 +257,26 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
   kp.xdir = xd ? 1 : 0;
   if (stage == MHY_STAGE_CELLS) {
-    const KernelFn k = sweepKernel(x->sweepWave, xd, x->o.nbmat, x->o.pp == 1);
     const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny) : gridFor(x->ncell);
-    timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, x->stream, kp); });
+    if (x->mixed) {
+      const KernelFnF k = sweepWaveKernel<float>(xd, x->o.nbmat, x->o.pp == 1);
+      timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
+            [&] { hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, x->stream, retype<float>(kp)); });
+    } else {
+      const KernelFn k = sweepKernel(x->sweepWave, xd, x->o.nbmat, x->o.pp == 1);
+      timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, x->stream, kp); });
+    }
   } else if (x->sweepWave && xd) {
     // along y the per-node kernel is faster (a 2-column wave strip reads
     // 16-byte row segments)
     const dim3 grid = dualWaveGrid(true, x->o.nx, x->o.ny);
-    timed(x, K_DUAL_X, [&] { hipLaunchKernelGGL(k_dual_wave<true>, grid, dim3(kBlock), 0, x->stream, kp); });
+    timed(x, K_DUAL_X, [&] { launchR(x, k_dual_wave<double, true>, k_dual_wave<float, true>, grid, dim3(kBlock), kp); });
   } else if (xd) {
-    timed(x, K_DUAL_X, [&] { hipLaunchKernelGGL(k_dual<true>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
+    timed(x, K_DUAL_X,
+          [&] { launchR(x, k_dual<double, true>, k_dual<float, true>, gridFor(x->nnode), dim3(kBlock), kp); });
   } else {
     timed(x, K_DUAL_Y,
-          [&] { hipLaunchKernelGGL(k_dual<false>, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp); });
+          [&] { launchR(x, k_dual<double, false>, k_dual<float, false>, gridFor(x->nnode), dim3(kBlock), kp); });
   }
   return checkLaunch(x, "sweep");
 }
@@ -248,9 +292,10 @@ int phaseFinal(mhy_ctx* x) {
     return MHY_OK;
   }
   KP kp = makeKP(x);
-  static const KernelFn kf[3] = {k_remap_final<1>, k_remap_final<2>, k_remap_final<3>};
-  const KernelFn k = kf[x->o.nbmat - 1];
-  timed(x, K_FINAL, [&] { hipLaunchKernelGGL(k, gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp); });
+  static const KernelFn kf[3] = {k_remap_final<double, 1>, k_remap_final<double, 2>, k_remap_final<double, 3>};
+  static const KernelFnF kff[3] = {k_remap_final<float, 1>, k_remap_final<float, 2>, k_remap_final<float, 3>};
+  const int m = x->o.nbmat - 1;
+  timed(x, K_FINAL, [&] { launchR(x, kf[m], kff[m], gridFor(x->ncell), dim3(kBlock), kp); });
   return checkLaunch(x, "final");
 }
 
@@ -283,6 +328,30 @@ int runIteration(mhy_ctx* x) {
 int upload(mhy_ctx* x, double* dst, const std::vector<double>& src) {
   HIPCHK(x, hipMemcpy(dst, src.data(), sizeof(double) * src.size(), hipMemcpyHostToDevice));
   return MHY_OK;
+}
+
+// upload of a bulk field (rounded to FP32 in the mixed form)
+int uploadR(mhy_ctx* x, double* dst, const std::vector<double>& src) {
+  if (!x->mixed) return upload(x, dst, src);
+  std::vector<float> h(src.begin(), src.end());
+  HIPCHK(x, hipMemcpy(dst, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  return MHY_OK;
+}
+
+// true for the fields stored with the bulk type R (FP32 in the mixed form)
+bool isBulkField(int field) {
+  switch (field) {
+    case MHY_CELL_MASS:
+    case MHY_NODE_COORD_X:
+    case MHY_NODE_COORD_Y:
+      return false;
+    default: break;
+  }
+  if (field >= MHY_MAT_FRACVOL && field < MHY_MAT_FRACVOL + MHY_NBMATMAX * MHY_MAT_STRIDE) {
+    const int id = field - ((field - MHY_MAT_FRACVOL) / MHY_MAT_STRIDE) * MHY_MAT_STRIDE;
+    return id != MHY_MAT_FRACVOL && id != MHY_MAT_MASS;
+  }
+  return true;
 }
 
 // device pointer and length of a field id (state n of the reference)
@@ -351,7 +420,10 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   };
   if (c.schema != MHY_SCHEMA_VNR) return bail(fail(x, MHY_ERR_ARG, "only the VNR scheme is supported"));
   if (c.projection_order > 2) return bail(fail(x, MHY_ERR_ARG, "projection order 3 is not supported"));
-  if (c.precision != 0) return bail(fail(x, MHY_ERR_ARG, "only FP64 (precision 0) is supported"));
+  if (c.precision != 0 && c.precision != 1)
+    return bail(fail(x, MHY_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed FP32 / FP64)"));
+  x->mixed = (c.precision == 1);
+  if (x->mixed) x->sweepWave = true;  // the per-cell stencil sweeps are FP64 only
   if (c.sans_lagrange && !c.avec_projection)
     return bail(fail(x, MHY_ERR_ARG, "sansLagrange needs the projection"));
   std::string err;
@@ -392,28 +464,29 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   Fields& f = x->f;
   std::memset(&f, 0, sizeof(f));
 #define AD(ptr, n) ((ptr) = allocD(x, (n), e))
+#define AR(ptr, n) ((ptr) = allocR(x, (n), e))
   for (int s = 0; s < 2; ++s) {
-    AD(f.rho[s], C), AD(f.p[s], C), AD(f.q[s], C), AD(f.c[s], C), AD(f.divu[s], C), AD(f.e[s], C);
+    AR(f.rho[s], C), AR(f.p[s], C), AR(f.q[s], C), AR(f.c[s], C), AR(f.divu[s], C), AR(f.e[s], C);
     for (int k = 0; k < m; ++k) {
-      AD(f.rho_k[s][k], C), AD(f.p_k[s][k], C), AD(f.q_k[s][k], C), AD(f.c_k[s][k], C), AD(f.e_k[s][k], C);
+      AR(f.rho_k[s][k], C), AR(f.p_k[s][k], C), AR(f.q_k[s][k], C), AR(f.c_k[s][k], C), AR(f.e_k[s][k], C);
     }
-    AD(f.ux[s], N), AD(f.uy[s], N);
+    AR(f.ux[s], N), AR(f.uy[s], N);
   }
   if (o.mode == MODE_LAGRANGE) {
     for (int s = 0; s < 2; ++s) AD(f.X[s], N), AD(f.Y[s], N), AD(f.vol[s], C);
-    AD(f.outvx, N), AD(f.outvy, N);
+    AR(f.outvx, N), AR(f.outvy, N);
   } else {
     AD(f.vol[0], C);
     f.vol[1] = f.vol[0];  // Euler sub-volumes: constant
   }
-  for (int k = 0; k < m; ++k) AD(f.f_k[k], C), AD(f.mu_k[k], C), AD(f.M_k[k], C);
+  for (int k = 0; k < m; ++k) AD(f.f_k[k], C), AR(f.mu_k[k], C), AD(f.M_k[k], C);
   AD(f.M, C);
   f.mixte = allocI(x, C, e);
   f.pure = allocI(x, C, e);
-  AD(f.Mp, N);
+  AR(f.Mp, N);
   if (o.periodic_right) {
-    AD(f.rho0, C), AD(f.e0, C);
-    for (int k = 0; k < m; ++k) AD(f.rho_k0[k], C), AD(f.e_k0[k], C);
+    AR(f.rho0, C), AR(f.e0, C);
+    for (int k = 0; k < m; ++k) AR(f.rho_k0[k], C), AR(f.e_k0[k], C);
   }
   if (o.mode != MODE_LAGRANGE) {
     for (int q = 0; q < o.nc; ++q) AD(f.UL[q], C), AD(f.U1[q], C), AD(f.U2[q], C);
@@ -434,6 +507,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     f.sc = static_cast<Scalars*>(p);
   }
   AD(f.partials, 2 * kReduceBlocks);
+#undef AR
 #undef AD
   if (e != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, std::string("device allocation: ") + hipGetErrorString(e)));
 
@@ -442,14 +516,18 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   auto up = [&](double* d, const std::vector<double>& s) {
     if (u == MHY_OK) u = upload(x, d, s);
   };
-  up(f.rho[0], h.rho), up(f.p[0], h.p), up(f.c[0], h.c), up(f.e[0], h.e);
+  auto upR = [&](double* d, const std::vector<double>& s) {
+    if (u == MHY_OK) u = uploadR(x, d, s);
+  };
+  upR(f.rho[0], h.rho), upR(f.p[0], h.p), upR(f.c[0], h.c), upR(f.e[0], h.e);
   for (int k = 0; k < m; ++k) {
-    up(f.rho_k[0][k], h.rho_k[k]), up(f.p_k[0][k], h.p_k[k]), up(f.c_k[0][k], h.c_k[k]), up(f.e_k[0][k], h.e_k[k]);
-    up(f.f_k[k], h.f_k[k]), up(f.mu_k[k], h.mu_k[k]), up(f.M_k[k], h.M_k[k]);
+    upR(f.rho_k[0][k], h.rho_k[k]), upR(f.p_k[0][k], h.p_k[k]), upR(f.c_k[0][k], h.c_k[k]);
+    upR(f.e_k[0][k], h.e_k[k]);
+    up(f.f_k[k], h.f_k[k]), upR(f.mu_k[k], h.mu_k[k]), up(f.M_k[k], h.M_k[k]);
   }
   up(f.M, h.M);
   up(f.vol[0], h.vol);
-  up(f.ux[0], h.ux), up(f.uy[0], h.uy);
+  upR(f.ux[0], h.ux), upR(f.uy[0], h.uy);
   if (o.mode == MODE_LAGRANGE) {
     std::vector<double> X(N), Y(N);
     for (long j = 0; j <= c.ny; ++j)
@@ -458,11 +536,11 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
         X[p] = c.dx * (double)i, Y[p] = c.dy * (double)j;
       }
     up(f.X[0], X), up(f.Y[0], Y);
-    up(f.outvx, h.ux), up(f.outvy, h.uy);
+    upR(f.outvx, h.ux), upR(f.outvy, h.uy);
   }
   if (o.periodic_right) {
-    up(f.rho0, h.rho), up(f.e0, h.e);
-    for (int k = 0; k < m; ++k) up(f.rho_k0[k], h.rho_k[k]), up(f.e_k0[k], h.e_k[k]);
+    upR(f.rho0, h.rho), upR(f.e0, h.e);
+    for (int k = 0; k < m; ++k) upR(f.rho_k0[k], h.rho_k[k]), upR(f.e_k0[k], h.e_k[k]);
   }
   if (u != MHY_OK) return bail(u);
   if (hipMemcpy(f.mixte, h.mixte.data(), sizeof(int) * C, hipMemcpyHostToDevice) != hipSuccess ||
@@ -477,7 +555,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   // computeNodeMass of the prologue (VnrRemap.cc:341)
   {
     KP kp = makeKP(x);
-    hipLaunchKernelGGL(k_node_mass, gridFor(N), dim3(kBlock), 0, x->stream, kp);
+    launchR(x, k_node_mass<double>, k_node_mass<float>, gridFor(N), dim3(kBlock), kp);
     if (hipStreamSynchronize(x->stream) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, "node mass"));
   }
   x->init = HostInit();  // the host copy of the initial state is no longer needed
@@ -608,7 +686,7 @@ int mhy_get_field(mhy_ctx* x, int field, double* out, long long n) {
   }
   if (field == MHY_NODE_MASS) {
     KP kp = makeKP(x);
-    hipLaunchKernelGGL(k_node_mass, gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp);
+    launchR(x, k_node_mass<double>, k_node_mass<float>, gridFor(x->nnode), dim3(kBlock), kp);
     HIPCHK(x, hipStreamSynchronize(x->stream));
   }
   if ((field == MHY_NODE_COORD_X || field == MHY_NODE_COORD_Y) && x->o.mode != MODE_LAGRANGE) {
@@ -622,6 +700,12 @@ int mhy_get_field(mhy_ctx* x, int field, double* out, long long n) {
   double* p = fieldPtr(x, field, len);
   if (!p) return fail(x, MHY_ERR_ARG, "unknown field " + std::to_string(field));
   if (n != len) return fail(x, MHY_ERR_ARG, "size");
+  if (x->mixed && isBulkField(field)) {
+    std::vector<float> h(len);
+    HIPCHK(x, hipMemcpy(h.data(), p, sizeof(float) * len, hipMemcpyDeviceToHost));
+    for (long i = 0; i < len; ++i) out[i] = h[i];
+    return MHY_OK;
+  }
   HIPCHK(x, hipMemcpy(out, p, sizeof(double) * len, hipMemcpyDeviceToHost));
   return MHY_OK;
 }
@@ -633,6 +717,11 @@ int mhy_set_field(mhy_ctx* x, int field, const double* in, long long n) {
   if (!p || field == MHY_NODE_MASS) return fail(x, MHY_ERR_ARG, "field not settable");
   if (n != len) return fail(x, MHY_ERR_ARG, "size");
   HIPCHK(x, hipStreamSynchronize(x->stream));
+  if (x->mixed && isBulkField(field)) {
+    std::vector<float> h(in, in + len);
+    HIPCHK(x, hipMemcpy(p, h.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+    return MHY_OK;
+  }
   HIPCHK(x, hipMemcpy(p, in, sizeof(double) * len, hipMemcpyHostToDevice));
   return MHY_OK;
 }
@@ -644,7 +733,7 @@ int mhy_conservation(mhy_ctx* x, double* sums) {
   double* d = nullptr;
   HIPCHK(x, hipMalloc(&d, sizeof(double) * nb * nq * 4));
   KP kp = makeKP(x);
-  hipLaunchKernelGGL(k_conservation, dim3(nb), dim3(kBlock), 0, x->stream, kp, d);
+  launchR(x, k_conservation<double>, k_conservation<float>, dim3(nb), dim3(kBlock), kp, d);
   std::vector<double> h(nb * nq * 4);
   hipError_t e1 = hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, x->stream);
   hipError_t e2 = hipStreamSynchronize(x->stream);
@@ -671,38 +760,51 @@ const char* mhy_kernel_name(const mhy_ctx*, int k) { return (k >= 0 && k < K_COU
 
 // Algorithmic bytes of one launch: every input word the kernel must read
 // from HBM once and every output word it must write once (neighbour reuse
-// through the caches counted once).
+// through the caches counted once).  Bulk fields count 4 bytes in the mixed
+// form, everything else 8 (flags 4).
 int mhy_kernel_bytes(const mhy_ctx* x, int k, double* bytes) {
   if (!x || !bytes || k < 0 || k >= K_COUNT) return MHY_ERR_ARG;
   const double C = (double)x->ncell, N = (double)x->nnode, F = (double)x->nface;
   const int m = x->o.nbmat, nc = x->o.nc;
   const bool pp = x->o.pp == 1;
   const int meth = x->o.methode;
+  const double r = x->mixed ? 4.0 : 8.0, d = 8.0, i4 = 4.0;
+  // cell part of the final remap: U2 in registers or read; writes rho, e, q,
+  // p, c, mu_k, rho_k, e_k, p_k, c_k (bulk) and M, f_k, M_k, flags
+  const double finalWrites = C * r * (5 + 5 * m) + C * d * (1 + 2 * m) + C * i4 * 2;
   double w = 0;
   switch (k) {
-    case K_DT: w = C * 3 + N * 2; break;  // vol, c, p ; u
-    case K_NODE: w = N * 2 + C * 3 + N * 3; break;  // u_n ; p, q, M ; u_n+1, Mp
+    case K_DT: w = C * (d + 2 * r) + N * 2 * r; break;  // vol ; c, p ; u
+    case K_NODE: w = N * 2 * r + C * (2 * r + d) + N * 3 * r; break;  // u_n ; p, q ; M ; u_n+1, Mp
     case K_CELL:
-      // u_n+1 ; f,mu,M_k,rho_k,e_k,p_k,q_k (7m) + rho, divu, c, vol ; writes UL, VL, XcL, YcL, divu, q_k (+rL, qL)
-      w = N * 2 + C * (7 * m + 4) + C * (nc + 4 + m + (meth == 1) + pp);
+      // u_n+1 ; f, M_k, vol (FP64), mu, rho_k, e_k, p_k, q_k + rho, divu, c ;
+      // writes UL, VL, XcL, YcL (+ rL, qL) (FP64), divu, q_k
+      w = N * 2 * r + C * ((2 * m + 1) * d + (5 * m + 3) * r) + C * d * (nc + 3 + (meth == 1) + pp) + C * r * (1 + m);
       break;
     case K_SWEEP_X:
-    case K_SWEEP_Y:
-      // U_in, VL, Xc (+qL, flags) ; u ; writes U_out, face fluxes (or PB)
-      w = C * (nc + 2 + pp + 2 * (pp && x->o.pp_mixte)) + N * 2 + C * nc + (meth == 2 ? C * m : F * m);
+    case K_SWEEP_Y: {
+      // U_in, VL or qL, Xc (+ flags) ; u ; writes U_out, face fluxes (or PB)
+      const double in = C * d * (nc + 2) + C * i4 * 2 * (pp && x->o.pp_mixte) + N * 2 * r;
+      const double fl = (meth == 2 ? C * m : F * m) * d;
+      const double s1 = in + C * nc * d + fl;
+      // sweep 2 with the final remap fused: no U2 store, the final outputs
+      // instead; X and Y alternate between sweep 1 and sweep 2
+      const double s2 = x->fuseFinal ? in + finalWrites + fl : s1;
+      w = 0.5 * (s1 + s2);
       break;
+    }
     case K_DUAL_X:
     case K_DUAL_Y:
-      // face fluxes (or PB cell fluxes) + sweep 1: u, Mp ; UD1, XL2 (7 N)
-      // or sweep 2: UD1, XL2 ; u (6 N)
-      w = N * 7 + (meth == 2 ? C * m : F * m);
+      // face fluxes (or PB cell fluxes) + sweep 1: u, Mp ; UD1, XL2
+      // or sweep 2: UD1, XL2 ; u (average of the two)
+      w = (meth == 2 ? C * m : F * m) * d + 0.5 * (N * 3 * r + N * 4 * d) + 0.5 * (N * 4 * d + N * 2 * r);
       break;
     case K_FINAL:
-      // U2 (+ void EOS inputs) ; writes rho, e, q, p, c, M, flags, f, mu, M_k, rho_k, e_k, p_k, c_k
-      w = C * nc + C * (5 + 1 + 1 + 7 * m);
+      // U2 (+ void EOS inputs) ; final outputs
+      w = C * nc * d + finalWrites;
       break;
   }
-  *bytes = w * 8.0;
+  *bytes = w;
   return MHY_OK;
 }
 
@@ -806,16 +908,16 @@ enum BatchKernel { BK_DT = 0, BK_NODE, BK_CELL, BK_SWX, BK_SWY, BK_DUX, BK_DUY, 
 constexpr int kVariants = 16;  // cur (1 bit), sweep-1 (1), xdir (1), fuseFinal (1)
 constexpr unsigned kCand[] = {0u, 1024u, 2048u, 4096u, 8192u};  // 0: one workgroup per virtual block
 constexpr int kNCand = sizeof(kCand) / sizeof(kCand[0]);
-using BatchFn = void (*)(BatchLaunch);
-
-BatchFn sweepBatchKernel(bool xd, int m, bool pp) {
-  static const BatchFn t[2][3][2] = {
-      {{k_sweep_wave_batch<1, false, false>, k_sweep_wave_batch<1, true, false>},
-       {k_sweep_wave_batch<2, false, false>, k_sweep_wave_batch<2, true, false>},
-       {k_sweep_wave_batch<3, false, false>, k_sweep_wave_batch<3, true, false>}},
-      {{k_sweep_wave_batch<1, false, true>, k_sweep_wave_batch<1, true, true>},
-       {k_sweep_wave_batch<2, false, true>, k_sweep_wave_batch<2, true, true>},
-       {k_sweep_wave_batch<3, false, true>, k_sweep_wave_batch<3, true, true>}}};
+template <class R>
+void (*sweepBatchKernel(bool xd, int m, bool pp))(BatchLaunchT<R>) {
+  using Fn = void (*)(BatchLaunchT<R>);
+  static const Fn t[2][3][2] = {
+      {{k_sweep_wave_batch<R, 1, false, false>, k_sweep_wave_batch<R, 1, true, false>},
+       {k_sweep_wave_batch<R, 2, false, false>, k_sweep_wave_batch<R, 2, true, false>},
+       {k_sweep_wave_batch<R, 3, false, false>, k_sweep_wave_batch<R, 3, true, false>}},
+      {{k_sweep_wave_batch<R, 1, false, true>, k_sweep_wave_batch<R, 1, true, true>},
+       {k_sweep_wave_batch<R, 2, false, true>, k_sweep_wave_batch<R, 2, true, true>},
+       {k_sweep_wave_batch<R, 3, false, true>, k_sweep_wave_batch<R, 3, true, true>}}};
   return t[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
 }  // namespace
@@ -825,7 +927,8 @@ struct mhy_batch {
   hipStream_t stream = nullptr;
   int m = 0;
   bool pp = false;
-  KP* dTable = nullptr;
+  bool mixed = false;  // every problem in the mixed FP32 / FP64 form
+  void* dTable = nullptr;  // KPT<double> or KPT<float> entries
   unsigned* dStart = nullptr;  // BK_COUNT x (n+1)
   unsigned* dGx = nullptr;     // BK_COUNT x n
   unsigned total[BK_COUNT] = {};
@@ -849,29 +952,36 @@ int batchFail(mhy_batch* b, int code, const std::string& msg) {
   return code;
 }
 
-// one step of every problem; timing != 0 records events around each launch
-int batchEnqueueStep(mhy_batch* b, int cur, bool xty, bool timing) {
+// one step of every problem with bulk storage type R; timing != 0 records
+// events around each launch
+template <class R>
+int batchEnqueueStepR(mhy_batch* b, int cur, bool xty, bool timing) {
   const int n = (int)b->ctx.size();
-  auto launch = [&](int k, BatchFn fn, int variant) {
-    BatchLaunch bl{b->dTable, b->dStart + (size_t)k * (n + 1), b->dGx + (size_t)k * n, n, variant, b->total[k]};
+  auto launch = [&](int k, void (*fn)(BatchLaunchT<R>), int variant) {
+    BatchLaunchT<R> bl{static_cast<const KPT<R>*>(b->dTable), b->dStart + (size_t)k * (n + 1),
+                       b->dGx + (size_t)k * n, n, variant, b->total[k]};
     const unsigned grid = b->blocks[k] ? std::min(b->blocks[k], b->total[k]) : b->total[k];
     if (timing) (void)hipEventRecord(b->ev[2 * k], b->stream);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, b->stream, bl);
     if (timing) (void)hipEventRecord(b->ev[2 * k + 1], b->stream);
   };
-  launch(BK_DT, k_dt_reduce_batch, cur);
-  launch(BK_NODE, k_lagrange_node_batch, cur);
-  launch(BK_CELL, k_lagrange_cell_batch, cur);
+  launch(BK_DT, k_dt_reduce_batch<R>, cur);
+  launch(BK_NODE, k_lagrange_node_batch<R>, cur);
+  launch(BK_CELL, k_lagrange_cell_batch<R>, cur);
   for (int sweep = 1; sweep <= 2; ++sweep) {
     const bool xd = (sweep == 1) ? xty : !xty;
     const int fuse = (sweep == 2) ? 1 : 0;
     const int v = cur + 2 * (sweep - 1) + 4 * (xd ? 1 : 0);
-    launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel(xd, b->m, b->pp), v + 8 * fuse);
-    if (xd) launch(BK_DUX, k_dual_wave_batch<true>, v);
-    else launch(BK_DUY, k_dual_batch<false>, v);
+    launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R>(xd, b->m, b->pp), v + 8 * fuse);
+    if (xd) launch(BK_DUX, k_dual_wave_batch<R, true>, v);
+    else launch(BK_DUY, k_dual_batch<R, false>, v);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MHY_OK : batchFail(b, MHY_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+}
+
+int batchEnqueueStep(mhy_batch* b, int cur, bool xty, bool timing) {
+  return b->mixed ? batchEnqueueStepR<float>(b, cur, xty, timing) : batchEnqueueStepR<double>(b, cur, xty, timing);
 }
 
 // captures the two step parities of the lock-step ensemble as graphs:
@@ -916,11 +1026,13 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
     const mhy_config& c = cfgs[i];
     if (c.schema != MHY_SCHEMA_VNR || !c.avec_projection || c.sans_lagrange)
       return batchFail(b, MHY_ERR_ARG, "batch problems must use the VNR Lagrange + remap mode");
-    if (i > 0 && (c.nbmat != cfgs[0].nbmat || c.pp != cfgs[0].pp))
-      return batchFail(b, MHY_ERR_ARG, "batch problems must share the material count and the PenteBorne option");
+    if (i > 0 && (c.nbmat != cfgs[0].nbmat || c.pp != cfgs[0].pp || c.precision != cfgs[0].precision))
+      return batchFail(b, MHY_ERR_ARG,
+                       "batch problems must share the material count, the PenteBorne option and the precision");
   }
   b->m = cfgs[0].nbmat;
   b->pp = cfgs[0].pp == 1;
+  b->mixed = cfgs[0].precision == 1;
   if (const char* g = std::getenv("MAHYCO_BATCH_GRAPH")) b->useGraph = (std::string(g) != "0");
   if (const char* t = std::getenv("MAHYCO_BATCH_TUNE")) b->tune = (std::string(t) != "0");
   b->ctx.resize(n, nullptr);
@@ -937,7 +1049,9 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
   if (hipStreamCreate(&b->stream) != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, "stream");
   for (auto& e : b->ev)
     if (hipEventCreate(&e) != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, "events");
-  // launch parameters of every problem and step variant
+  // launch parameters of every problem and step variant (KPT<float> has the
+  // same layout, so the mixed form uploads the same entries)
+  static_assert(sizeof(KPT<float>) == sizeof(KP), "parameter blocks differ");
   std::vector<KP> table((size_t)kVariants * n);
   for (int v = 0; v < kVariants; ++v)
     for (int i = 0; i < n; ++i) {

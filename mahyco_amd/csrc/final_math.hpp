@@ -11,11 +11,14 @@ namespace mhy {
 // first error wins (MHY_ERR_*), read by the host after the step
 __device__ __forceinline__ void raiseErr(Scalars* sc, int code) { atomicCAS(&sc->err, 0, code); }
 
-// U: the 3m+1 live components of the remapped vector U2 of cell c
-template <int M>
-__device__ __forceinline__ void finalCell(const KP& kp, long c, const double* U) {
+// U: the 3m+1 live components of the remapped vector U2 of cell c.
+// Arithmetic is FP64 in both forms: the volume fractions near 0 and 1, their
+// threshold tests and the masses stay FP64; R only sets the storage of the
+// bulk outputs.
+template <class R, int M>
+__device__ __forceinline__ void finalCell(const KPT<R>& kp, long c, const double* U) {
   const Opts& o = kp.o;
-  const Fields& f = kp.f;
+  const auto& f = kp.f;
   const int cur = kp.cur, nxt = kp.nxt;
   const double thr = o.threshold;
   const double vol = o.dx * o.dy;  // m_euler_volume (Init.cc:427)
@@ -98,19 +101,19 @@ __device__ __forceinline__ void finalCell(const KP& kp, long c, const double* U)
       pk[k] = (g - 1.0) * rk[k] * ek[k];
       ck[k] = sqrt(g * (g - 1.0) * ek[k]);
     } else if (o.eos[k] == MHY_EOS_VOID) {
-      pk[k] = f.p_k[cur][k][c];
-      ck[k] = f.c_k[cur][k][c];
-      ek[k] = f.e_k[cur][k][c];
+      pk[k] = (double)f.p_k[cur][k][c];
+      ck[k] = (double)f.c_k[cur][k][c];
+      ek[k] = (double)f.e_k[cur][k][c];
       e1 = 0.0;
     } else {
-      pk[k] = f.p_k[nxt][k][c];
-      ck[k] = f.c_k[nxt][k][c];
+      pk[k] = (double)f.p_k[nxt][k][c];
+      ck[k] = (double)f.c_k[nxt][k][c];
     }
   }
   // computePressionMoyenne
   double p1 = 0.0, c1 = 0.0;
   const bool needStale = !(rho1 > 0.);
-  if (needStale) c1 = f.c[nxt][c];
+  if (needStale) c1 = (double)f.c[nxt][c];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     p1 += fr[k] * pk[k];

@@ -32,7 +32,8 @@ __device__ __forceinline__ double loadCoherent(const double* p) {
 __device__ __forceinline__ void raise(Scalars* sc, int code) { atomicCAS(&sc->err, 0, code); }
 
 // node coordinates at the step start
-__device__ __forceinline__ D2 nodeX(const KP& kp, long j, long i) {
+template <class R>
+__device__ __forceinline__ D2 nodeX(const KPT<R>& kp, long j, long i) {
   if (kp.o.mode == MODE_LAGRANGE) {
     long p = j * (kp.o.nx + 1) + i;
     return {kp.f.X[kp.cur][p], kp.f.Y[kp.cur][p]};
@@ -46,14 +47,16 @@ __device__ __forceinline__ D2 nodeX(const KP& kp, long j, long i) {
 // max-pressure reduction of PhaseLagrange.cc:55-61.  Min and max ignore NaN
 // values like the reference's reducers; the last block to finish combines
 // the per-block partials.
-// b: block index among the nb blocks working on this problem
-__device__ __forceinline__ void dtReduceBody(const KP& kp, unsigned b, unsigned nb) {
+// b: block index among the nb blocks working on this problem.  The
+// reduction is FP64 in both forms (FP32 inputs are widened).
+template <class R>
+__device__ __forceinline__ void dtReduceBody(const KPT<R>& kp, unsigned b, unsigned nb) {
   const Opts& o = kp.o;
-  const Fields& f = kp.f;
+  const auto& f = kp.f;
   const long nx = o.nx;
   double vmin = DBL_MAX, pm = -DBL_MAX;
-  const double* ux = f.ux[kp.cur];
-  const double* uy = f.uy[kp.cur];
+  const R* ux = f.ux[kp.cur];
+  const R* uy = f.uy[kp.cur];
   for (long c = (long)b * blockDim.x + threadIdx.x; c < kp.ncell; c += (long)nb * blockDim.x) {
     const long j = c / nx, i = c - j * nx;
     const long p0 = j * (nx + 1) + i;
@@ -63,7 +66,7 @@ __device__ __forceinline__ void dtReduceBody(const KP& kp, unsigned b, unsigned 
       const double a = ux[nd[s]], b = uy[nd[s]];
       uc += sqrt(a * a + b * b) * 0.25;
     }
-    const double v = o.cfl * sqrt(f.vol[kp.cur][c]) / (o.cfl_euler * uc + f.c[kp.cur][c]);
+    const double v = o.cfl * sqrt(f.vol[kp.cur][c]) / (o.cfl_euler * uc + (double)f.c[kp.cur][c]);
     vmin = smin(vmin, v);
     const double pc = f.p[kp.cur][c];
     pm = (pc > pm) ? pc : pm;
@@ -109,10 +112,14 @@ __device__ __forceinline__ void dtReduceBody(const KP& kp, unsigned b, unsigned 
   }
 }
 
-__global__ __launch_bounds__(256) void k_dt_reduce(KP kp) { dtReduceBody(kp, blockIdx.x, gridDim.x); }
+template <class R>
+__global__ __launch_bounds__(256) void k_dt_reduce(KPT<R> kp) {
+  dtReduceBody(kp, blockIdx.x, gridDim.x);
+}
 
 // sansLagrange: deltat_nplus1 = deltat_n (VnrRemap.cc:143-146)
-__global__ void k_dt_hold(KP kp) {
+template <class R>
+__global__ void k_dt_hold(KPT<R> kp) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     Scalars* sc = kp.f.sc;
     sc->dt[kp.nxt] = sc->dt[kp.cur];
@@ -124,10 +131,12 @@ __global__ void k_dt_hold(KP kp) {
 // (PhaseLagrange.cc:31-48), then updateVelocity (PhaseLagrange.cc:167-197),
 // updateVelocityBoundaryConditions (ConditionsLimites.cc:12-185) and
 // updatePosition (PhaseLagrange.cc:203-208).  In advection mode the velocity
-// is carried over unchanged (VnrRemap.cc:152-154).
-__device__ __forceinline__ void lagrangeNodeBody(const KP& kp, unsigned b) {
+// is carried over unchanged (VnrRemap.cc:152-154).  Arithmetic is FP64 in
+// both forms.
+template <class R>
+__device__ __forceinline__ void lagrangeNodeBody(const KPT<R>& kp, unsigned b) {
   const Opts& o = kp.o;
-  const Fields& f = kp.f;
+  const auto& f = kp.f;
   const long p = (long)b * blockDim.x + threadIdx.x;
   if (p >= kp.nnode) return;
   const long nx = o.nx, ny = o.ny;
@@ -141,7 +150,7 @@ __device__ __forceinline__ void lagrangeNodeBody(const KP& kp, unsigned b) {
     if (hc[q]) msum = msum + f.M[cc[q]], ++cnt;
   const double Mp = msum / (double)cnt;
   f.Mp[p] = Mp;
-  const double uxn = f.ux[kp.cur][p], uyn = f.uy[kp.cur][p];
+  const double uxn = (double)f.ux[kp.cur][p], uyn = (double)f.uy[kp.cur][p];
   if (o.mode == MODE_ADVECTION) {
     f.ux[kp.nxt][p] = uxn;
     f.uy[kp.nxt][p] = uyn;
@@ -160,7 +169,7 @@ __device__ __forceinline__ void lagrangeNodeBody(const KP& kp, unsigned b) {
   if (hc[3]) Cn[3] = lpcNpc(X0, nodeX(kp, j, i - 1), nodeX(kp, j - 1, i));
   double pq[4];
   for (int q = 0; q < 4; ++q)
-    if (hc[q]) pq[q] = f.p[kp.cur][cc[q]] + f.q[kp.cur][cc[q]];
+    if (hc[q]) pq[q] = (double)f.p[kp.cur][cc[q]] + (double)f.q[kp.cur][cc[q]];
   const bool inner = (i != 0 && j != 0 && i != nx && j != ny);
   D2 un = {uxn, uyn};
   D2 u1;
@@ -172,7 +181,7 @@ __device__ __forceinline__ void lagrangeNodeBody(const KP& kp, unsigned b) {
   } else {
     // sides in the reference's launch order, then corners; a side that is
     // neither symmetry nor imposed velocity leaves the buffer unchanged
-    u1 = {f.ux[kp.nxt][p], f.uy[kp.nxt][p]};
+    u1 = {(double)f.ux[kp.nxt][p], (double)f.uy[kp.nxt][p]};
     auto side = [&](int bc, D2 axis, const double* val) {
       if (bc == MHY_BC_SYMMETRY) {
         D2 r = {0.0, 0.0};
@@ -205,17 +214,25 @@ __device__ __forceinline__ void lagrangeNodeBody(const KP& kp, unsigned b) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_lagrange_node(KP kp) { lagrangeNodeBody(kp, blockIdx.x); }
+template <class R>
+__global__ __launch_bounds__(256) void k_lagrange_node(KPT<R> kp) {
+  lagrangeNodeBody(kp, blockIdx.x);
+}
 
 // Cell kernel: updateCellPos, computeSubVol, updateRho (+ periodic cells),
 // computeTau, computeDivU, computeArtificialViscosity, updateEnergy,
 // computeEOS, computePressionMoyenne (PhaseLagrange.cc:214-485,
 // ConditionsLimites.cc:186-209), then the cell part of the remap
 // preparation: computeVariablesForRemap and computeCellQuantitesForRemap
-// (PrepareRemap.cc:65-176).
-__device__ __forceinline__ void lagrangeCellBody(const KP& kp, unsigned b) {
+// (PrepareRemap.cc:65-176).  Arithmetic is FP64 in both forms; the mixed
+// form takes the remap masses from the FP64 material masses M_k (equal to
+// mu_k V rho in exact arithmetic), so that mass stays conserved to FP64
+// round-off whatever the FP32 density storage (periodic cells, whose
+// density is reset to its initial value, keep mu_k V rho).
+template <class R>
+__device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
   const Opts& o = kp.o;
-  const Fields& f = kp.f;
+  const auto& f = kp.f;
   const long c = (long)b * blockDim.x + threadIdx.x;
   if (c >= kp.ncell) return;
   const int m = o.nbmat;
@@ -234,7 +251,7 @@ __device__ __forceinline__ void lagrangeCellBody(const KP& kp, unsigned b) {
         X[s] = {f.X[nxt][p], f.Y[nxt][p]};
       } else {
         const D2 e = eulerNode(o, jj[s], ii[s]);
-        X[s] = {e.x + f.ux[nxt][p] * dt1, e.y + f.uy[nxt][p] * dt1};
+        X[s] = {e.x + (double)f.ux[nxt][p] * dt1, e.y + (double)f.uy[nxt][p] * dt1};
       }
     }
   }
@@ -254,20 +271,25 @@ __device__ __forceinline__ void lagrangeCellBody(const KP& kp, unsigned b) {
                                    : (o.mode == MODE_FULL ? 0.0 : f.rho_k[nxt][k][c]);
     rho1 += fk[k] * rk1[k];
   }
-  if (o.periodic_right && (i == 0 || i == nx - 1)) {
+  const bool periodicCell = o.periodic_right && (i == 0 || i == nx - 1);
+  if (periodicCell) {
     const long L = j * nx;  // updatePeriodicBoundaryConditions
-    rho1 = f.rho0[L];
+    rho1 = (double)f.rho0[L];
     for (int k = 0; k < m; ++k) rk1[k] = f.rho_k0[k][L];
   }
 
   double q1 = 0.0, e1 = 0.0;
   if (o.mode != MODE_ADVECTION) {
-    const double rho0 = f.rho[cur][c];
+    // mixed form: the step-start densities from the FP64 masses and volume,
+    // so that 1/rho1 - 1/rho0 (divided by dt in div u) carries no FP32
+    // rounding of the stored density
+    const double vol0 = f.vol[cur][c];
+    const double rho0 = kMixed<R> ? f.M[c] / vol0 : (double)f.rho[cur][c];
     const double tau1 = 0.5 * (1.0 / rho1 + 1.0 / rho0);
     const double div1 = 1.0 / dt1 * (1.0 / rho1 - 1.0 / rho0) / tau1;
     if (f.divu[cur][c] < 0.0) {
-      const double r0 = f.vol[cur][c];
-      q1 = 1.0 / tau1 * (-0.5 * sqrt(r0) * f.c[cur][c] * div1 + (o.gamma + 1) / 2.0 * r0 * div1 * div1);
+      const double r0 = vol0;
+      q1 = 1.0 / tau1 * (-0.5 * sqrt(r0) * (double)f.c[cur][c] * div1 + (o.gamma + 1) / 2.0 * r0 * div1 * div1);
     } else {
       q1 = 0.0;
     }
@@ -278,16 +300,17 @@ __device__ __forceinline__ void lagrangeCellBody(const KP& kp, unsigned b) {
     bool bad = false;
     for (int k = 0; k < m; ++k) {
       ek1[k] = 0.0;
-      const double r1 = rk1[k], r0 = f.rho_k[cur][k][c];
+      const double r1 = rk1[k];
+      const double r0 = (kMixed<R> && fk[k] > o.threshold) ? f.M_k[k][c] / (fk[k] * vol0) : (double)f.rho_k[cur][k][c];
       if ((r1 > o.threshold) && (r0 > o.threshold)) {
-        const double qn = f.q_k[cur][k][c];
+        const double qn = (double)f.q_k[cur][k][c];
         double pseudo = 0.0;
         if ((o.pseudo_centree == 1) && ((qk1[k] + qn) * (1.0 / r1 - 1.0 / r0) > 0.)) pseudo = 0.5 * (qk1[k] + qn);
         if (o.pseudo_centree == 0) pseudo = qk1[k];
         const double den = 1 + 0.5 * (o.gammap[k] - 1.0) * r1 * (1.0 / r1 - 1.0 / r0);
-        const double num = f.e_k[cur][k][c] - (0.5 * f.p_k[cur][k][c] + pseudo) * (1.0 / r1 - 1.0 / r0);
+        const double num = (double)f.e_k[cur][k][c] - (0.5 * (double)f.p_k[cur][k][c] + pseudo) * (1.0 / r1 - 1.0 / r0);
         ek1[k] = num / den;
-        e1 += f.mu_k[k][c] * ek1[k];
+        e1 += (double)f.mu_k[k][c] * ek1[k];
         if (fabs(ek1[k]) > 5. && o.abort_energy) bad = true;
       }
     }
@@ -311,14 +334,14 @@ __device__ __forceinline__ void lagrangeCellBody(const KP& kp, unsigned b) {
     // computePressionMoyenne (PhaseLagrange.cc:464-485)
     double p1 = 0.0, c1 = 0.0;
     const bool needStale = !(rho1 > 0.);
-    if (needStale) c1 = f.c[nxt][c];
+    if (needStale) c1 = (double)f.c[nxt][c];
     for (int k = 0; k < m; ++k) {
       p1 += fk[k] * pk1[k];
       if (needStale) c1 = smax(c1, ck1[k]);
     }
     if (rho1 > 0.) c1 = sqrt(o.gammap[0] * (o.gammap[0] - 1.0) * e1);
     for (int k = 0; k < m; ++k)
-      if (o.eos[k] == MHY_EOS_VOID) e1 += f.mu_k[k][c] * ek1[k];
+      if (o.eos[k] == MHY_EOS_VOID) e1 += (double)f.mu_k[k][c] * ek1[k];
     f.divu[nxt][c] = div1;
     for (int k = 0; k < m; ++k) f.q_k[nxt][k][c] = qk1[k];
     if (o.mode == MODE_LAGRANGE) {
@@ -340,15 +363,21 @@ __device__ __forceinline__ void lagrangeCellBody(const KP& kp, unsigned b) {
     // advection: the energy swap of VnrRemap.cc:170 makes the remap read the
     // current energy; the pseudo buffer of the n+1 slot is read as left by
     // the previous remap of that slot
-    e1 = f.e[cur][c];
-    q1 = f.q[nxt][c];
+    e1 = (double)f.e[cur][c];
+    q1 = (double)f.q[nxt][c];
   }
   // computeVariablesForRemap (PrepareRemap.cc:96-176), live components only
   for (int k = 0; k < m; ++k) {
-    const double mu = f.mu_k[k][c];
     f.UL[k][c] = fk[k] * V;
-    f.UL[m + k][c] = mu * V * rho1;
-    f.UL[2 * m + k][c] = mu * V * rho1 * e1;
+    if (kMixed<R> && !periodicCell) {
+      const double Mk = f.M_k[k][c];
+      f.UL[m + k][c] = Mk;
+      f.UL[2 * m + k][c] = Mk * e1;
+    } else {
+      const double mu = f.mu_k[k][c];
+      f.UL[m + k][c] = mu * V * rho1;
+      f.UL[2 * m + k][c] = mu * V * rho1 * e1;
+    }
   }
   f.UL[3 * m][c] = q1 * V;
   f.VL[c] = V;
@@ -365,24 +394,42 @@ __device__ __forceinline__ void lagrangeCellBody(const KP& kp, unsigned b) {
   f.YcL[c] = r.y;
 }
 
-__global__ __launch_bounds__(256) void k_lagrange_cell(KP kp) { lagrangeCellBody(kp, blockIdx.x); }
+template <class R>
+__global__ __launch_bounds__(256) void k_lagrange_cell(KPT<R> kp) {
+  lagrangeCellBody(kp, blockIdx.x);
+}
 
 // ---------------------------------------------------------------- batches
 // One launch serves every problem of an ensemble: the virtual blocks of all
 // problems are numbered consecutively (BatchLaunch::start) and a fixed
 // number of workgroups walks them.
 
-__global__ __launch_bounds__(256) void k_dt_reduce_batch(BatchLaunch bl) {
-  batchForEach(bl, [&](const KP& kp, unsigned b, unsigned, unsigned nb) {
+template <class R>
+__global__ __launch_bounds__(256) void k_dt_reduce_batch(BatchLaunchT<R> bl) {
+  batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned nb) {
     dtReduceBody(kp, b, nb);
     __syncthreads();  // the shared scratch is reused by the next virtual block
   });
 }
-__global__ __launch_bounds__(256) void k_lagrange_node_batch(BatchLaunch bl) {
-  batchForEach(bl, [&](const KP& kp, unsigned b, unsigned, unsigned) { lagrangeNodeBody(kp, b); });
+template <class R>
+__global__ __launch_bounds__(256) void k_lagrange_node_batch(BatchLaunchT<R> bl) {
+  batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeNodeBody(kp, b); });
 }
-__global__ __launch_bounds__(256) void k_lagrange_cell_batch(BatchLaunch bl) {
-  batchForEach(bl, [&](const KP& kp, unsigned b, unsigned, unsigned) { lagrangeCellBody(kp, b); });
+template <class R>
+__global__ __launch_bounds__(256) void k_lagrange_cell_batch(BatchLaunchT<R> bl) {
+  batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeCellBody(kp, b); });
 }
+
+#define MHY_LAGRANGE_INST(R)                                                 \
+  template __global__ void k_dt_reduce<R>(KPT<R>);                           \
+  template __global__ void k_dt_hold<R>(KPT<R>);                             \
+  template __global__ void k_lagrange_node<R>(KPT<R>);                       \
+  template __global__ void k_lagrange_cell<R>(KPT<R>);                       \
+  template __global__ void k_dt_reduce_batch<R>(BatchLaunchT<R>);            \
+  template __global__ void k_lagrange_node_batch<R>(BatchLaunchT<R>);        \
+  template __global__ void k_lagrange_cell_batch<R>(BatchLaunchT<R>);
+MHY_LAGRANGE_INST(double)
+MHY_LAGRANGE_INST(float)
+#undef MHY_LAGRANGE_INST
 
 }  // namespace mhy

@@ -1,9 +1,11 @@
 // Nodal (dual) momentum remap of each sweep, the final remap with the EOS,
 // and the node-mass / conservation helpers.  The cell sweeps are in
-// sweep_cell.hip (and sweep.hip).
+// sweep_wave.hip and sweep_cell.hip.
 //
-// Layout: structure-of-arrays FP64 in HBM, one thread per node (cell) with
-// the stencil along the sweep read through the L1/L2 caches.
+// Layout: structure-of-arrays in HBM (bulk fields of storage type R: FP64 or
+// FP32; masses, fluxes and the nodal momentum FP64), one thread per node
+// (cell) with the stencil along the sweep read through the L1/L2 caches.
+// Arithmetic is FP64 in both forms.
 #include "common.hpp"
 #include "final_math.hpp"
 #include "kernels.hpp"
@@ -63,10 +65,10 @@ __device__ __forceinline__ double limitedGradient(int lim, double gp, double gm,
 // Nodal momentum remap of one sweep (remap/DualPhaseRemap1.cc:18-136,
 // remap/DualPhaseRemap2.cc:18-127 with remap/UtilesDualRemap.cc and
 // UtilesRemap-Impl.h:8-97).  XD: along x (left/right nodes).
-template <bool XD>
-__device__ __forceinline__ void dualBody(const KP& kp, unsigned b) {
+template <class R, bool XD>
+__device__ __forceinline__ void dualBody(const KPT<R>& kp, unsigned b) {
   const Opts& o = kp.o;
-  const Fields& f = kp.f;
+  const auto& f = kp.f;
   const long p = (long)b * blockDim.x + threadIdx.x;
   if (p >= kp.nnode) return;
   const int m = o.nbmat;
@@ -90,7 +92,7 @@ __device__ __forceinline__ void dualBody(const KP& kp, unsigned b) {
     }
     const long q = p + (d - 3) * stride;
     if (kp.sweep == 1) {
-      const double ux1 = f.ux[kp.nxt][q], uy1 = f.uy[kp.nxt][q];
+      const double ux1 = (double)f.ux[kp.nxt][q], uy1 = (double)f.uy[kp.nxt][q];
       dpx[d] = ux1, dpy[d] = uy1;
       const long jq = q / (nx + 1), iq = q - jq * (nx + 1);
       const D2 e = eulerNode(o, jq, iq);
@@ -106,7 +108,7 @@ __device__ __forceinline__ void dualBody(const KP& kp, unsigned b) {
   if (kp.sweep == 1) {
     // Lagrange coordinate of this node along the second sweep's axis
     const D2 e = eulerNode(o, j, i);
-    f.XL2[p] = XD ? e.y + f.uy[kp.nxt][p] * dt1 : e.x + f.ux[kp.nxt][p] * dt1;
+    f.XL2[p] = XD ? e.y + (double)f.uy[kp.nxt][p] * dt1 : e.x + (double)f.ux[kp.nxt][p] * dt1;
   }
   // limited nodal gradient at slot d (2..4) for velocity components 0, 1
   auto gradAt = [&](int d, double* g) {
@@ -216,18 +218,14 @@ __device__ __forceinline__ void dualBody(const KP& kp, unsigned b) {
   }
 }
 
-template <bool XD>
-__global__ __launch_bounds__(256) void k_dual(KP kp) {
-  dualBody<XD>(kp, blockIdx.x);
+template <class R, bool XD>
+__global__ __launch_bounds__(256) void k_dual(KPT<R> kp) {
+  dualBody<R, XD>(kp, blockIdx.x);
 }
-template <bool XD>
-__global__ __launch_bounds__(256) void k_dual_batch(BatchLaunch bl) {
-  batchForEach(bl, [&](const KP& kp, unsigned local, unsigned, unsigned) { dualBody<XD>(kp, local); });
+template <class R, bool XD>
+__global__ __launch_bounds__(256) void k_dual_batch(BatchLaunchT<R> bl) {
+  batchForEach(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) { dualBody<R, XD>(kp, local); });
 }
-template __global__ void k_dual<true>(KP);
-template __global__ void k_dual<false>(KP);
-template __global__ void k_dual_batch<true>(BatchLaunch);
-template __global__ void k_dual_batch<false>(BatchLaunch);
 
 // The same nodal remap with one lane per node and the neighbour values taken
 // by wave shuffles.  A wave covers a strip of nodes along the sweep: along x,
@@ -246,11 +244,11 @@ __device__ __forceinline__ double dshfl(double v, int src) { return __shfl(v, sr
 }  // namespace
 
 // b: linear block index (strips numbered row-major along x, as the sweeps)
-template <bool XD>
-__device__ __forceinline__ void dualWaveBody(const KP& kp, unsigned b) {
+template <class R, bool XD>
+__device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   constexpr int S = XD ? 1 : kDYCols;  // lane distance of the along-sweep neighbour
   const Opts& o = kp.o;
-  const Fields& f = kp.f;
+  const auto& f = kp.f;
   const int m = o.nbmat;
   const long nx = o.nx, ny = o.ny;
   const int lane = threadIdx.x & (kDW - 1), wave = threadIdx.x >> 6;
@@ -283,7 +281,7 @@ __device__ __forceinline__ void dualWaveBody(const KP& kp, unsigned b) {
   double dpx0 = 0.0, dpy0 = 0.0, xl0 = 0.0;
   if (in) {
     if (kp.sweep == 1) {
-      const double ux1 = f.ux[kp.nxt][p], uy1 = f.uy[kp.nxt][p];
+      const double ux1 = (double)f.ux[kp.nxt][p], uy1 = (double)f.uy[kp.nxt][p];
       dpx0 = ux1, dpy0 = uy1;
       const D2 e = eulerNode(o, j, i);
       xl0 = XD ? e.x + ux1 * dt1 : e.y + uy1 * dt1;
@@ -421,18 +419,14 @@ __device__ __forceinline__ void dualWaveBody(const KP& kp, unsigned b) {
   }
 }
 
-template <bool XD>
-__global__ __launch_bounds__(256) void k_dual_wave(KP kp) {
-  dualWaveBody<XD>(kp, blockIdx.x);
+template <class R, bool XD>
+__global__ __launch_bounds__(256) void k_dual_wave(KPT<R> kp) {
+  dualWaveBody<R, XD>(kp, blockIdx.x);
 }
-template <bool XD>
-__global__ __launch_bounds__(256) void k_dual_wave_batch(BatchLaunch bl) {
-  batchForEach(bl, [&](const KP& kp, unsigned local, unsigned, unsigned) { dualWaveBody<XD>(kp, local); });
+template <class R, bool XD>
+__global__ __launch_bounds__(256) void k_dual_wave_batch(BatchLaunchT<R> bl) {
+  batchForEach(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) { dualWaveBody<R, XD>(kp, local); });
 }
-template __global__ void k_dual_wave<true>(KP);
-template __global__ void k_dual_wave<false>(KP);
-template __global__ void k_dual_wave_batch<true>(BatchLaunch);
-template __global__ void k_dual_wave_batch<false>(BatchLaunch);
 
 // launch grid of the wave dual kernels (256-thread blocks of 4 waves)
 dim3 dualWaveGrid(bool xd, long nx, long ny) {
@@ -446,22 +440,20 @@ dim3 dualWaveGrid(bool xd, long nx, long ny) {
 
 // Vnr::remapVariables, cells (PhaseRemapFinal.cc:23-205), then computeEOS
 // and computePressionMoyenne (PhaseLagrange.cc:392-485) on the remapped state.
-template <int M>
-__global__ __launch_bounds__(256) void k_remap_final(KP kp) {
+template <class R, int M>
+__global__ __launch_bounds__(256) void k_remap_final(KPT<R> kp) {
   constexpr int NC = 3 * M + 1;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= kp.ncell) return;
   double U[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) U[q] = kp.f.U2[q][c];
-  finalCell<M>(kp, c, U);
+  finalCell<R, M>(kp, c, U);
 }
-template __global__ void k_remap_final<1>(KP);
-template __global__ void k_remap_final<2>(KP);
-template __global__ void k_remap_final<3>(KP);
 
 // computeNodeMass for output (PhaseLagrange.cc:31-48)
-__global__ void k_node_mass(KP kp) {
+template <class R>
+__global__ void k_node_mass(KPT<R> kp) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= kp.nnode) return;
   const long nx = kp.o.nx, ny = kp.o.ny;
@@ -478,8 +470,9 @@ __global__ void k_node_mass(KP kp) {
 // Conservation sums with Neumaier compensation: out[0] mass, out[1]
 // internal energy (sum M*e), out[2+k] material masses.  Per-block partial
 // pairs (sum, compensation) are combined on the host.
-__global__ __launch_bounds__(256) void k_conservation(KP kp, double* out) {
-  const Fields& f = kp.f;
+template <class R>
+__global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
+  const auto& f = kp.f;
   const int m = kp.o.nbmat;
   const int nq = 2 + m;
   double s[2 + NMAT], comp[2 + NMAT];
@@ -493,7 +486,7 @@ __global__ __launch_bounds__(256) void k_conservation(KP kp, double* out) {
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < kp.ncell; c += (long)gridDim.x * blockDim.x) {
     const double M = f.M[c];
     add(0, M);
-    add(1, M * f.e[kp.cur][c]);
+    add(1, M * (double)f.e[kp.cur][c]);
     for (int k = 0; k < m; ++k) add(2 + k, f.M_k[k][c]);
   }
   __shared__ double sh[256];
@@ -517,5 +510,23 @@ __global__ __launch_bounds__(256) void k_conservation(KP kp, double* out) {
     }
   }
 }
+
+#define MHY_REMAP_INST(R)                                              \
+  template __global__ void k_dual<R, true>(KPT<R>);                    \
+  template __global__ void k_dual<R, false>(KPT<R>);                   \
+  template __global__ void k_dual_batch<R, true>(BatchLaunchT<R>);     \
+  template __global__ void k_dual_batch<R, false>(BatchLaunchT<R>);    \
+  template __global__ void k_dual_wave<R, true>(KPT<R>);               \
+  template __global__ void k_dual_wave<R, false>(KPT<R>);              \
+  template __global__ void k_dual_wave_batch<R, true>(BatchLaunchT<R>); \
+  template __global__ void k_dual_wave_batch<R, false>(BatchLaunchT<R>); \
+  template __global__ void k_remap_final<R, 1>(KPT<R>);                \
+  template __global__ void k_remap_final<R, 2>(KPT<R>);                \
+  template __global__ void k_remap_final<R, 3>(KPT<R>);                \
+  template __global__ void k_node_mass<R>(KPT<R>);                     \
+  template __global__ void k_conservation<R>(KPT<R>, double*);
+MHY_REMAP_INST(double)
+MHY_REMAP_INST(float)
+#undef MHY_REMAP_INST
 
 }  // namespace mhy

@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
 #pragma unroll
   for (int q = 0; q < NC; ++q) Unew[q] = Uin[q][c] - red[q];
   if (kp.fuseFinal) {
-    finalCell<M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
+    finalCell<double, M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
   } else {
 #pragma unroll
     for (int q = 0; q < NC; ++q) Uout[q][c] = Unew[q];

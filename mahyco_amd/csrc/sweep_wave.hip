@@ -11,6 +11,16 @@
 // fluxes Ar and Av) once, and takes the neighbour values it needs with wave
 // shuffles: no LDS is allocated.  Operation order follows the reference, so
 // FP64 results are bitwise those of the CPU path.
+//
+// R is the bulk storage type.  With R = float (mixed form) the limited-slope
+// sweeps compute Phi, slopes and face values in FP32, with centroid offsets
+// formed in FP64 before rounding (so the FP32 slopes see cell-size
+// differences, not absolute coordinates); the PenteBorne sweeps keep FP64
+// arithmetic, because their swept integrals of nearly emptied materials
+// cancel catastrophically in FP32.  Each face flux is formed in FP64 and
+// accumulated into the FP64 remap vector with a compensated (TwoSum) update.
+#include <type_traits>
+
 #include "final_math.hpp"
 #include "kernels.hpp"
 #include "sweep_math.hpp"
@@ -28,15 +38,16 @@ constexpr int kYRows = kWave / kYCols;
 constexpr int kYOut = kYRows - 2;  // rows updated per wave along y
 
 // end-of-Lagrange node position (updatePosition, PhaseLagrange.cc:203-208)
-__device__ __forceinline__ D2 lagNodeW(const KP& kp, double dt1, long j, long i) {
+template <class R>
+__device__ __forceinline__ D2 lagNodeW(const KPT<R>& kp, double dt1, long j, long i) {
   const long p = j * (kp.o.nx + 1) + i;
   const D2 e = eulerNode(kp.o, j, i);
-  return {e.x + kp.f.ux[kp.nxt][p] * dt1, e.y + kp.f.uy[kp.nxt][p] * dt1};
+  return {e.x + (double)kp.f.ux[kp.nxt][p] * dt1, e.y + (double)kp.f.uy[kp.nxt][p] * dt1};
 }
 
 // width across the sweep faces (HvLagrange, PhaseRemap1.cc:57-71, :104-119)
-template <bool XD>
-__device__ __forceinline__ double widthW(const KP& kp, double dt1, long j, long i) {
+template <bool XD, class R>
+__device__ __forceinline__ double widthW(const KPT<R>& kp, double dt1, long j, long i) {
   double h = 0.;
   if (XD) {  // bottom then top face
     h += 0.5 * d2norm(d2sub(lagNodeW(kp, dt1, j, i + 1), lagNodeW(kp, dt1, j, i)));
@@ -49,19 +60,30 @@ __device__ __forceinline__ double widthW(const KP& kp, double dt1, long j, long 
 }
 
 __device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, kWave); }
+__device__ __forceinline__ float shfl(float v, int src) { return __shfl(v, src, kWave); }
 __device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, kWave); }
+
+// s + v with the rounding error of the addition added to comp (TwoSum)
+__device__ __forceinline__ void twoSumAcc(double& s, double& comp, double v) {
+  const double t = s + v;
+  const double bv = t - s;
+  comp += (s - (t - bv)) + (v - bv);
+  s = t;
+}
 
 }  // namespace
 
 // b: linear block index.  Along x the strips of all rows are numbered
 // row-major and each wave of the block takes the next one; along y a block
 // covers 16 columns x 14 rows, blocks ordered column-group-major per row band.
-template <int M, bool PP, bool XD>
-__device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
+template <class R, int M, bool PP, bool XD>
+__device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
+  // type of the slope / limiter / PenteBorne arithmetic
+  using T = typename std::conditional<kMixed<R> && !PP, float, double>::type;
   constexpr int NC = 3 * M + 1;
   constexpr int S = XD ? 1 : kYCols;  // lane distance of the along-sweep neighbour
   const Opts& o = kp.o;
-  const Fields& f = kp.f;
+  const auto& f = kp.f;
   const long nx = o.nx, ny = o.ny;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
 
@@ -99,23 +121,24 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
   const bool pb = PP && o.methode == 2;
 
   // one cell of this lane's line at along index b: Phi, centroid, width, flags
-  auto loadCell = [&](long b, double* ph, double& x, double& h, int& flg) {
+  auto loadCell = [&](long b, T* ph, double& x, T& h, int& flg) {
     const long jb = XD ? t : b, ib = XD ? b : t;
     const long cb = jb * nx + ib;
     double u[NC];
     const int nload = PP ? 3 * M : NC;
 #pragma unroll
     for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cb] : 0.0;
-    cellPhi<M, PP, double>(u, PP ? 1.0 : f.VL[cb], PP ? f.qL[cb] : 0.0, ph);
+    cellPhi<M, PP, T>(u, PP ? 1.0 : f.VL[cb], PP ? f.qL[cb] : 0.0, ph);
     x = XD ? f.XcL[cb] : f.YcL[cb];
-    h = needH ? widthW<XD>(kp, dt1, jb, ib) : 0.0;
+    h = needH ? T(widthW<XD>(kp, dt1, jb, ib)) : T(0.0);
     flg = 0;
-    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<M, double>(ph);
+    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<M, T>(ph);
   };
-  double phi[NC], xc = 0.0, H = 0.0;
+  T phi[NC], H = T(0.0);
+  double xc = 0.0;
   int fl = 0;
 #pragma unroll
-  for (int q = 0; q < NC; ++q) phi[q] = 0.0;
+  for (int q = 0; q < NC; ++q) phi[q] = T(0.0);
   if (in) loadCell(a, phi, xc, H, fl);
 
   // neighbours along the sweep from the adjacent lanes; the halo lanes at
@@ -123,22 +146,27 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
   // neighbour takes this cell's values (PhaseRemap1.cc:157-158)
   const int lm = (lane - S >= 0) ? lane - S : lane, lp = (lane + S < kWave) ? lane + S : lane;
   const bool inM = (a >= 1), inP = (a + 1 < na);
-  double phm[NC], php[NC];
+  T phm[NC], php[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
-    const double vm = shfl(phi[q], lm), vp = shfl(phi[q], lp);
+    const T vm = shfl(phi[q], lm), vp = shfl(phi[q], lp);
     phm[q] = inM ? vm : phi[q];
     php[q] = inP ? vp : phi[q];
   }
   double xcm = shfl(xc, lm), xcp = shfl(xc, lp);
   xcm = inM ? xcm : xc, xcp = inP ? xcp : xc;
-  double Hm = shfl(H, lm), Hp = shfl(H, lp);
+  T Hm = shfl(H, lm), Hp = shfl(H, lp);
   Hm = inM ? Hm : H, Hp = inP ? Hp : H;
   int fm = shfl(fl, lm), fp = shfl(fl, lp);
   fm = inM ? fm : fl, fp = inP ? fp : fl;
   if (in && slot == 0 && inM) loadCell(a - 1, phm, xcm, Hm, fm);
   if (in && slot == lastSlot && inP) loadCell(a + 1, php, xcp, Hp, fp);
-  Nbr<double> n{phm, phi, php, xcm, xc, xcp, inM, inP, H, Hm, Hp, false};
+  // FP64: absolute centroids (the reference's differences); FP32: offsets
+  // from this cell's centroid, formed in FP64
+  constexpr bool kF32 = sizeof(T) == 4;
+  const T xcmT = kF32 ? T(xcm - xc) : T(xcm), xcT = kF32 ? T(0.0) : T(xc);
+  const T xcpT = kF32 ? T(xcp - xc) : T(xcp);
+  Nbr<T> n{phm, phi, php, xcmT, xcT, xcpT, inM, inP, H, Hm, Hp, false};
   if (mixteOpt) n.vp = ((fl >> 8) == 0) && fl == fp && fl == fm;
 
   // normal velocities of the minus / plus faces (PrepareRemap.cc:50-62)
@@ -146,8 +174,8 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
     long n1, n2;
     if (XD) n1 = j * (nx + 1) + b, n2 = (j + 1) * (nx + 1) + b;
     else n1 = b * (nx + 1) + i, n2 = b * (nx + 1) + i + 1;
-    const double* uu = XD ? f.ux[kp.nxt] : f.uy[kp.nxt];
-    return 0.0 + ((0.0 + uu[n1]) + uu[n2]) * 0.5;
+    const R* uu = XD ? f.ux[kp.nxt] : f.uy[kp.nxt];
+    return 0.0 + ((0.0 + (double)uu[n1]) + (double)uu[n2]) * 0.5;
   };
   const double vm = in ? fnv(a) : 0.0, vp = in ? fnv(a + 1) : 0.0;
   // Euler lengths of the minus / plus faces (Init.cc:528-533)
@@ -160,25 +188,25 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
   const double Lm = faceLen(a), Lp = faceLen(a + 1);
   const bool hasM = inM, hasP = inP;
 
-  double phiM[NC], phiP[NC];
+  T phiM[NC], phiP[NC];
 #pragma unroll
-  for (int q = 0; q < NC; ++q) phiM[q] = phiP[q] = 0.0;
+  for (int q = 0; q < NC; ++q) phiM[q] = phiP[q] = T(0.0);
   if (!PP) {
     // own limited slope, then the upwind value of each face
     // (UtilesRemap-Impl.h:491-501); back/front cells: x: left/right,
     // y: above/below
-    double g[NC], gm[NC], gp[NC];
+    T g[NC], gm[NC], gp[NC];
 #pragma unroll
-    for (int q = 0; q < NC; ++q) g[q] = in ? gradQ<XD, double>(o, n, q) : 0.0;
+    for (int q = 0; q < NC; ++q) g[q] = in ? gradQ<XD, T>(o, n, q) : T(0.0);
 #pragma unroll
     for (int q = 0; q < NC; ++q) gm[q] = shfl(g[q], lm), gp[q] = shfl(g[q], lp);
     // face between along-cells lo (lower index) and hi
-    auto faceValue = [&](long b, const double* pLo, const double* gLo, double xLo, const double* pHi,
-                         const double* gHi, double xHi, double v, double* out) {
+    auto faceValue = [&](long b, const T* pLo, const T* gLo, double xLo, const T* pHi, const T* gHi, double xHi,
+                         double v, T* out) {
       const double dxl = XD ? xHi - xLo : xLo - xHi;  // front - back
       const bool upLo = XD ? (v * dxl > 0.0) : !(v * dxl > 0.0);
       const double xf = XD ? 0.5 * (o.dx * (double)b + o.dx * (double)b) : 0.5 * (o.dy * (double)b + o.dy * (double)b);
-      const double s = 0.0 + (xf - (upLo ? xLo : xHi));
+      const T s = T(0.0 + (xf - (upLo ? xLo : xHi)));
 #pragma unroll
       for (int q = 0; q < NC; ++q) out[q] = upLo ? pLo[q] + gLo[q] * s : pHi[q] + gHi[q] * s;
     };
@@ -186,25 +214,25 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
     if (hasP) faceValue(a + 1, phi, g, xc, php, gp, xcp, vp, phiP);
   } else {
     // PenteBorne: phi_f = Av(minus cell) - Ar(plus cell) (PhaseRemap1.cc:361-364)
-    double Ar[NC], Av[NC], Dual[M];
+    T Ar[NC], Av[NC], Dual[M];
 #pragma unroll
-    for (int q = 0; q < NC; ++q) Ar[q] = Av[q] = 0.0;
+    for (int q = 0; q < NC; ++q) Ar[q] = Av[q] = T(0.0);
 #pragma unroll
-    for (int k = 0; k < M; ++k) Dual[k] = 0.0;
+    for (int k = 0; k < M; ++k) Dual[k] = T(0.0);
     const bool formed = o.order > 1;  // the fluxes are only formed for order > 1
-    if (formed && in) ppFluxes<M, XD, double>(o, n, vm, vp, dtn, pb && output, Ar, Av, Dual);
+    if (formed && in) ppFluxes<M, XD, T>(o, n, T(vm), T(vp), T(dtn), pb && output, Ar, Av, Dual);
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
-      const double avm = shfl(Av[q], lm), arp = shfl(Ar[q], lp);
+      const T avm = shfl(Av[q], lm), arp = shfl(Ar[q], lp);
       if (formed) {
-        phiM[q] = hasM ? avm - Ar[q] : 0.0;
-        phiP[q] = hasP ? Av[q] - arp : 0.0;
+        phiM[q] = hasM ? avm - Ar[q] : T(0.0);
+        phiP[q] = hasP ? Av[q] - arp : T(0.0);
       }
     }
     if (pb && output) {
       const double w = (Lm + Lp) * 0.5;
 #pragma unroll
-      for (int k = 0; k < M; ++k) f.DF[k][c] = formed ? Dual[k] * w : 0.0;
+      for (int k = 0; k < M; ++k) f.DF[k][c] = formed ? (double)Dual[k] * w : 0.0;
     }
   }
   if (!output) return;
@@ -213,14 +241,15 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
   // minus face and +G through its plus face
   const double coefM = PP ? Lm : vm * Lm * dtn;
   const double coefP = PP ? Lp : vp * Lp * dtn;
-  double Gm[NC], red[NC];
+  double Gm[NC], Gp[NC], red[NC], Fb[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
-    Gm[q] = coefM * phiM[q];
-    const double Gp = coefP * phiP[q];
+    Gm[q] = coefM * (double)phiM[q];
+    Gp[q] = coefP * (double)phiP[q];
     red[q] = 0.0;
+    Fb[q] = 0.0;
     if (hasM) red[q] = red[q] + (-Gm[q]);
-    if (hasP) red[q] = red[q] + Gp;
+    if (hasP) red[q] = red[q] + Gp[q];
   }
   if (o.flux_any) {
     // remap/BoundaryFluxes.cc:14-120: flux through one face of the cell with
@@ -238,17 +267,30 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
       const double sgn = viaMinus ? -1.0 : 1.0;
 #pragma unroll
       for (int q = 0; q < NC; ++q) {
-        const double ph = oppOk ? (viaMinus ? phiP[q] : phiM[q]) : 0.0;
-        const double Fb = PP ? (sgn * Lb) * ph : (((sgn * vb) * Lb) * dtn) * ph;
-        red[q] = red[q] + Fb;
+        const double ph = oppOk ? (double)(viaMinus ? phiP[q] : phiM[q]) : 0.0;
+        Fb[q] = PP ? (sgn * Lb) * ph : (((sgn * vb) * Lb) * dtn) * ph;
+        red[q] = red[q] + Fb[q];
       }
     }
   }
   double Unew[NC];
+  if (kMixed<R>) {
+    // U - (-Gm + Gp + Fb), compensated: the FP64 result is the correctly
+    // rounded sum up to a final rounding, whatever the flux magnitudes
 #pragma unroll
-  for (int q = 0; q < NC; ++q) Unew[q] = Uin[q][c] - red[q];
+    for (int q = 0; q < NC; ++q) {
+      double sum = Uin[q][c], comp = 0.0;
+      if (hasM) twoSumAcc(sum, comp, Gm[q]);
+      if (hasP) twoSumAcc(sum, comp, -Gp[q]);
+      twoSumAcc(sum, comp, -Fb[q]);
+      Unew[q] = sum + comp;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) Unew[q] = Uin[q][c] - red[q];
+  }
   if (kp.fuseFinal) {
-    finalCell<M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
+    finalCell<R, M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
   } else {
 #pragma unroll
     for (int q = 0; q < NC; ++q) Uout[q][c] = Unew[q];
@@ -269,28 +311,34 @@ __device__ __forceinline__ void sweepWaveBody(const KP& kp, unsigned b) {
   }
 }
 
-template <int M, bool PP, bool XD>
-__global__ __launch_bounds__(256) void k_sweep_wave(KP kp) {
-  sweepWaveBody<M, PP, XD>(kp, blockIdx.x);
+template <class R, int M, bool PP, bool XD>
+__global__ __launch_bounds__(256) void k_sweep_wave(KPT<R> kp) {
+  sweepWaveBody<R, M, PP, XD>(kp, blockIdx.x);
 }
 
-// all problems of an ensemble in one launch (BatchLaunch, common.hpp)
-template <int M, bool PP, bool XD>
-__global__ __launch_bounds__(256) void k_sweep_wave_batch(BatchLaunch bl) {
-  batchForEach(bl, [&](const KP& kp, unsigned local, unsigned, unsigned) { sweepWaveBody<M, PP, XD>(kp, local); });
+// all problems of an ensemble in one launch (BatchLaunchT, common.hpp)
+template <class R, int M, bool PP, bool XD>
+__global__ __launch_bounds__(256) void k_sweep_wave_batch(BatchLaunchT<R> bl) {
+  batchForEach(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) {
+    sweepWaveBody<R, M, PP, XD>(kp, local);
+  });
 }
 
-#define MHY_SWEEP_WAVE_INST(M, PP)                                      \
-  template __global__ void k_sweep_wave<M, PP, true>(KP);                \
-  template __global__ void k_sweep_wave<M, PP, false>(KP);               \
-  template __global__ void k_sweep_wave_batch<M, PP, true>(BatchLaunch); \
-  template __global__ void k_sweep_wave_batch<M, PP, false>(BatchLaunch);
-MHY_SWEEP_WAVE_INST(1, false)
-MHY_SWEEP_WAVE_INST(1, true)
-MHY_SWEEP_WAVE_INST(2, false)
-MHY_SWEEP_WAVE_INST(2, true)
-MHY_SWEEP_WAVE_INST(3, false)
-MHY_SWEEP_WAVE_INST(3, true)
+#define MHY_SWEEP_WAVE_INST(R, M, PP)                                          \
+  template __global__ void k_sweep_wave<R, M, PP, true>(KPT<R>);               \
+  template __global__ void k_sweep_wave<R, M, PP, false>(KPT<R>);              \
+  template __global__ void k_sweep_wave_batch<R, M, PP, true>(BatchLaunchT<R>); \
+  template __global__ void k_sweep_wave_batch<R, M, PP, false>(BatchLaunchT<R>);
+#define MHY_SWEEP_WAVE_INST_R(R)    \
+  MHY_SWEEP_WAVE_INST(R, 1, false) \
+  MHY_SWEEP_WAVE_INST(R, 1, true)  \
+  MHY_SWEEP_WAVE_INST(R, 2, false) \
+  MHY_SWEEP_WAVE_INST(R, 2, true)  \
+  MHY_SWEEP_WAVE_INST(R, 3, false) \
+  MHY_SWEEP_WAVE_INST(R, 3, true)
+MHY_SWEEP_WAVE_INST_R(double)
+MHY_SWEEP_WAVE_INST_R(float)
+#undef MHY_SWEEP_WAVE_INST_R
 #undef MHY_SWEEP_WAVE_INST
 
 // launch grid of the wave sweeps (256-thread blocks of 4 waves)
