@@ -267,7 +267,10 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
       const double sgn = viaMinus ? -1.0 : 1.0;
 #pragma unroll
       for (int q = 0; q < NC; ++q) {
-        const double ph = oppOk ? (double)(viaMinus ? phiP[q] : phiM[q]) : 0.0;
+        // values, not a conditional lvalue: a select between the two arrays'
+        // addresses would keep them in scratch memory
+        const double pP = phiP[q], pM = phiM[q];
+        const double ph = oppOk ? (viaMinus ? pP : pM) : 0.0;
         Fb[q] = PP ? (sgn * Lb) * ph : (((sgn * vb) * Lb) * dtn) * ph;
         red[q] = red[q] + Fb[q];
       }
