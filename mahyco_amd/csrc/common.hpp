@@ -52,6 +52,14 @@ MHY_HD D2 lpcNpc(D2 xp, D2 xpPlus, D2 xpMinus) {
   double l2 = d2norm(n2);
   return d2add(d2mul(d2div(n1, l1), l1), d2mul(d2div(n2, l2), l2));
 }
+// The same corner normal on the Euler grid.  Every half edge there is
+// axis-aligned, so each half-edge vector has one zero component and
+// |n| = sqrt(fl(a*a)) = |a| exactly (binary floating point), n/|n| = +-1 and
+// (n/|n|)*|n| = n: the sum of the two raw half-edge vectors is bitwise the
+// value of lpcNpc, without its 2 square roots and 4 divisions.
+MHY_HD D2 lpcNpcAxis(D2 xp, D2 xpPlus, D2 xpMinus) {
+  return {0.5 * (xpPlus.y - xp.y) + 0.5 * (xp.y - xpMinus.y), 0.5 * (xp.x - xpPlus.x) + 0.5 * (xpMinus.x - xp.x)};
+}
 // includes/Freefunctions.h:60-65
 MHY_HD D2 symmetricVector(D2 v, D2 sigma) { return d2sub(d2mul(sigma, 2.0 * d2dot(v, sigma)), v); }
 
@@ -156,6 +164,21 @@ KPT<R> retype(const KP& kp) {
   memcpy(&r, &kp, sizeof(r));
   return r;
 }
+
+// Logical block of this workgroup.  The hardware hands consecutive
+// workgroup ids to the 8 XCDs in turn, and each XCD has its own L2; the
+// renumbering gives every XCD one contiguous range of logical blocks (a band
+// of the mesh), so that the cache lines two neighbouring blocks share are
+// fetched into one L2 only.  It is a permutation of [0, gridDim.x).  Used by
+// the y sweeps, whose blocks cover 16-column groups (64 bytes of an FP32
+// row); measured slower for the kernels that stream whole rows.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcdBlock(unsigned b, unsigned nblocks) {
+  const unsigned q = nblocks / kXcds, r = nblocks % kXcds;
+  const unsigned k = b % kXcds, i = b / kXcds;
+  return k * q + (k < r ? k : r) + i;
+}
+__device__ __forceinline__ unsigned xcdBlock() { return xcdBlock(blockIdx.x, gridDim.x); }
 
 // One launch over an ensemble of problems: the launch parameters of problem
 // p for step variant v are table[v * nprob + p]; the virtual blocks of problem

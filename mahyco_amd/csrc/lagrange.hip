@@ -163,10 +163,17 @@ __device__ __forceinline__ void lagrangeNodeBody(const KPT<R>& kp, unsigned b) {
   // the node and its two neighbours along the cell's edges
   const D2 X0 = nodeX(kp, j, i);
   D2 Cn[4];
-  if (hc[0]) Cn[0] = lpcNpc(X0, nodeX(kp, j, i + 1), nodeX(kp, j + 1, i));
-  if (hc[1]) Cn[1] = lpcNpc(X0, nodeX(kp, j + 1, i), nodeX(kp, j, i - 1));
-  if (hc[2]) Cn[2] = lpcNpc(X0, nodeX(kp, j - 1, i), nodeX(kp, j, i + 1));
-  if (hc[3]) Cn[3] = lpcNpc(X0, nodeX(kp, j, i - 1), nodeX(kp, j - 1, i));
+  if (o.mode == MODE_LAGRANGE) {
+    if (hc[0]) Cn[0] = lpcNpc(X0, nodeX(kp, j, i + 1), nodeX(kp, j + 1, i));
+    if (hc[1]) Cn[1] = lpcNpc(X0, nodeX(kp, j + 1, i), nodeX(kp, j, i - 1));
+    if (hc[2]) Cn[2] = lpcNpc(X0, nodeX(kp, j - 1, i), nodeX(kp, j, i + 1));
+    if (hc[3]) Cn[3] = lpcNpc(X0, nodeX(kp, j, i - 1), nodeX(kp, j - 1, i));
+  } else {  // fixed Euler grid: axis-aligned edges
+    if (hc[0]) Cn[0] = lpcNpcAxis(X0, eulerNode(o, j, i + 1), eulerNode(o, j + 1, i));
+    if (hc[1]) Cn[1] = lpcNpcAxis(X0, eulerNode(o, j + 1, i), eulerNode(o, j, i - 1));
+    if (hc[2]) Cn[2] = lpcNpcAxis(X0, eulerNode(o, j - 1, i), eulerNode(o, j, i + 1));
+    if (hc[3]) Cn[3] = lpcNpcAxis(X0, eulerNode(o, j, i - 1), eulerNode(o, j - 1, i));
+  }
   double pq[4];
   for (int q = 0; q < 4; ++q)
     if (hc[q]) pq[q] = (double)f.p[kp.cur][cc[q]] + (double)f.q[kp.cur][cc[q]];

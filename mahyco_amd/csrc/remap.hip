@@ -110,14 +110,23 @@ __device__ __forceinline__ void dualBody(const KPT<R>& kp, unsigned b) {
     const D2 e = eulerNode(o, j, i);
     f.XL2[p] = XD ? e.y + (double)f.uy[kp.nxt][p] * dt1 : e.x + (double)f.ux[kp.nxt][p] * dt1;
   }
+  // divided differences between slots k and k+1 (k = 1..4), each formed
+  // once: the plus side of slot d is the minus side of slot d+1
+  double D0[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, D1[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (o.order > 1) {
+#pragma unroll
+    for (int k = 1; k <= 4; ++k) {
+      if (!in[k] || !in[k + 1]) continue;
+      D0[k] = (dpx[k + 1] - dpx[k]) / (xl[k + 1] - xl[k]);
+      D1[k] = (dpy[k + 1] - dpy[k]) / (xl[k + 1] - xl[k]);
+    }
+  }
   // limited nodal gradient at slot d (2..4) for velocity components 0, 1
   auto gradAt = [&](int d, double* g) {
     g[0] = g[1] = 0.0;
     if (o.order <= 1 || !in[d - 1] || !in[d + 1]) return;
-    const double gr0 = (dpx[d + 1] - dpx[d]) / (xl[d + 1] - xl[d]);
-    const double gl0 = (dpx[d] - dpx[d - 1]) / (xl[d] - xl[d - 1]);
-    const double gr1 = (dpy[d + 1] - dpy[d]) / (xl[d + 1] - xl[d]);
-    const double gl1 = (dpy[d] - dpy[d - 1]) / (xl[d] - xl[d - 1]);
+    const double gr0 = D0[d], gl0 = D0[d - 1];
+    const double gr1 = D1[d], gl1 = D1[d - 1];
     const double h0 = 0.5 * (xl[d + 1] - xl[d - 1]);
     double hm, hp;
     if (!in[d - 2]) {
@@ -311,14 +320,19 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   inS[0] = (a - 3 >= 0), inS[6] = (a + 3 <= na);
   dpx[0] = dpy[0] = xl[0] = dpx[6] = dpy[6] = xl[6] = 0.0;
 
-  // own limited nodal gradient (UtilesRemap-Impl.h:8-97)
+  // own limited nodal gradient (UtilesRemap-Impl.h:8-97).  Each lane forms
+  // the divided differences towards its minus neighbour; those towards the
+  // plus neighbour are the next lane's (the same operands).
+  const int lmS = (lane - S >= 0) ? lane - S : lane, lpS = (lane + S < kDW) ? lane + S : lane;
+  double gl0 = 0.0, gl1 = 0.0;
+  if (o.order > 1) {
+    gl0 = (dpx[3] - dpx[2]) / (xl[3] - xl[2]);
+    gl1 = (dpy[3] - dpy[2]) / (xl[3] - xl[2]);
+  }
+  const double gr0 = dshfl(gl0, lpS), gr1 = dshfl(gl1, lpS);
   double g0[2] = {0.0, 0.0};
   if (in && o.order > 1 && inS[2] && inS[4]) {
     const int d = 3;
-    const double gr0 = (dpx[d + 1] - dpx[d]) / (xl[d + 1] - xl[d]);
-    const double gl0 = (dpx[d] - dpx[d - 1]) / (xl[d] - xl[d - 1]);
-    const double gr1 = (dpy[d + 1] - dpy[d]) / (xl[d + 1] - xl[d]);
-    const double gl1 = (dpy[d] - dpy[d - 1]) / (xl[d] - xl[d - 1]);
     const double h0 = 0.5 * (xl[d + 1] - xl[d - 1]);
     double hm, hp;
     if (!inS[d - 2]) {
@@ -332,7 +346,6 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
     g0[0] = limitedGradient(o.limiter, gr0, gl0, dpx[d], dpx[d + 1], dpx[d - 1], h0, hp, hm);
     g0[1] = limitedGradient(o.limiter, gr1, gl1, dpy[d], dpy[d + 1], dpy[d - 1], h0, hp, hm);
   }
-  const int lmS = (lane - S >= 0) ? lane - S : lane, lpS = (lane + S < kDW) ? lane + S : lane;
   double gM[2], gP[2];
   gM[0] = dshfl(g0[0], lmS), gM[1] = dshfl(g0[1], lmS);
   gP[0] = dshfl(g0[0], lpS), gP[1] = dshfl(g0[1], lpS);

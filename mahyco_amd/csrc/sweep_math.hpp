@@ -202,6 +202,9 @@ struct Nbr {
   bool inM, inP;               // the minus / plus neighbour exists (inner face)
   T h0, hm, hp;                // widths (PenteBorne and generalised limiters)
   bool vp;                     // pure neighbourhood (PenteBorne mixte option)
+  // divided differences at the minus / plus faces when the caller has them
+  // (a neighbour's minus face is this cell's plus face); null: formed here
+  const T *dM = nullptr, *dP = nullptr;
 };
 
 // Divided differences at the minus / plus faces (gradPhiFace, PhaseRemap1.cc:
@@ -261,7 +264,8 @@ __device__ __forceinline__ void finishPP(const Opts& o, bool pureVariant, const 
 template <bool XD, class T>
 __device__ __forceinline__ T gradQ(const Opts& o, const Nbr<T>& n, int q) {
   if (o.order <= 1) return T(0.0);  // gradients are only formed for order > 1
-  const T gM = faceDiffM<XD, T>(n, q), gP = faceDiffP<XD, T>(n, q);
+  const T gM = n.dM ? n.dM[q] : faceDiffM<XD, T>(n, q);
+  const T gP = n.dP ? n.dP[q] : faceDiffP<XD, T>(n, q);
   const int lim = (o.pp == 1 && n.vp) ? o.limiter_pure : o.limiter;
   const T gp = XD ? gP : gM, gm = XD ? gM : gP;
   return limitedGradient<T>(lim, gp, gm, n.phip[q], n.phim[q], n.h0, n.hp, n.hm);

@@ -3,13 +3,15 @@
 // remap/PhaseRemap1.cc, remap/PhaseRemap2.cc), one lane per cell, with the
 // neighbours' values exchanged inside the 64-wide wavefront.
 //
-// A wave covers a strip of cells along the sweep direction plus one halo
-// cell at each end: along x, 64 consecutive cells of a row (62 updated);
-// along y, 4 columns x 16 rows (14 rows updated), so that every row access
+// A wave covers a strip of cells along the sweep direction plus two halo
+// cells at each end: along x, 64 consecutive cells of a row (60 updated);
+// along y, 4 columns x 16 rows (12 rows updated), so that every row access
 // still reads 4 consecutive cells and a 256-thread block reads 16.  Each lane
-// loads only its own cell, forms its own limited slope (or its own PenteBorne
-// fluxes Ar and Av) once, and takes the neighbour values it needs with wave
-// shuffles: no LDS is allocated.  Operation order follows the reference, so
+// loads only its own cell, forms the divided difference at its minus face,
+// its own limited slope (or its own PenteBorne fluxes Ar and Av) once, and
+// takes the neighbour values it needs with wave shuffles: no LDS is
+// allocated, and every slope an updated cell needs is formed inside the wave
+// (no lane loads a neighbour from memory on its own).  Operation order follows the reference, so
 // FP64 results are bitwise those of the CPU path.
 //
 // R is the bulk storage type.  With R = float (mixed form) the limited-slope
@@ -32,10 +34,11 @@ using namespace sw;
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kXOut = kWave - 2;   // cells updated per wave along x
-constexpr int kYCols = 4;          // columns per wave along y
+constexpr int kHalo = 2;                // halo cells at each end of a strip
+constexpr int kXOut = kWave - 2 * kHalo;   // cells updated per wave along x
+constexpr int kYCols = 4;               // columns per wave along y
 constexpr int kYRows = kWave / kYCols;
-constexpr int kYOut = kYRows - 2;  // rows updated per wave along y
+constexpr int kYOut = kYRows - 2 * kHalo;  // rows updated per wave along y
 
 // end-of-Lagrange node position (updatePosition, PhaseLagrange.cc:203-208)
 template <class R>
@@ -75,7 +78,7 @@ __device__ __forceinline__ void twoSumAcc(double& s, double& comp, double v) {
 
 // b: linear block index.  Along x the strips of all rows are numbered
 // row-major and each wave of the block takes the next one; along y a block
-// covers 16 columns x 14 rows, blocks ordered column-group-major per row band.
+// covers 16 columns x 12 rows, blocks ordered column-group-major per row band.
 template <class R, int M, bool PP, bool XD>
 __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   // type of the slope / limiter / PenteBorne arithmetic
@@ -95,14 +98,14 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     const long g = (long)b * 4 + wave;
     const long seg = g % nseg;
     slot = lane;
-    a = seg * kXOut - 1 + lane;
+    a = seg * kXOut - kHalo + lane;
     t = g / nseg;  // rows beyond ny have no cells
     na = nx, nt = ny;
   } else {
     const long gx = (nx + 4 * kYCols - 1) / (4 * kYCols);
     const long bx = b % gx, by = b / gx;
     slot = lane / kYCols;
-    a = by * kYOut - 1 + slot;
+    a = by * kYOut - kHalo + slot;
     t = bx * (4 * kYCols) + wave * kYCols + (lane % kYCols);
     na = ny, nt = nx;
   }
@@ -111,7 +114,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   const long j = XD ? t : a, i = XD ? a : t;
   const long c = j * nx + i;
   const int lastSlot = XD ? kWave - 1 : kYRows - 1;
-  const bool output = in && slot >= 1 && slot < lastSlot;
+  const bool output = in && slot >= kHalo && slot <= lastSlot - kHalo;
 
   double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
   double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
@@ -141,8 +144,8 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   for (int q = 0; q < NC; ++q) phi[q] = T(0.0);
   if (in) loadCell(a, phi, xc, H, fl);
 
-  // neighbours along the sweep from the adjacent lanes; the halo lanes at
-  // the ends of the strip read their outer neighbour from memory; a missing
+  // neighbours along the sweep from the adjacent lanes (the outermost halo
+  // lanes clamp to themselves: their slopes are not used); a missing
   // neighbour takes this cell's values (PhaseRemap1.cc:157-158)
   const int lm = (lane - S >= 0) ? lane - S : lane, lp = (lane + S < kWave) ? lane + S : lane;
   const bool inM = (a >= 1), inP = (a + 1 < na);
@@ -159,8 +162,6 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   Hm = inM ? Hm : H, Hp = inP ? Hp : H;
   int fm = shfl(fl, lm), fp = shfl(fl, lp);
   fm = inM ? fm : fl, fp = inP ? fp : fl;
-  if (in && slot == 0 && inM) loadCell(a - 1, phm, xcm, Hm, fm);
-  if (in && slot == lastSlot && inP) loadCell(a + 1, php, xcp, Hp, fp);
   // FP64: absolute centroids (the reference's differences); FP32: offsets
   // from this cell's centroid, formed in FP64
   constexpr bool kF32 = sizeof(T) == 4;
@@ -168,6 +169,24 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   const T xcpT = kF32 ? T(xcp - xc) : T(xcp);
   Nbr<T> n{phm, phi, php, xcmT, xcT, xcpT, inM, inP, H, Hm, Hp, false};
   if (mixteOpt) n.vp = ((fl >> 8) == 0) && fl == fp && fl == fm;
+  // divided differences (gradPhiFace): each lane forms its minus face's;
+  // its plus face is the next lane's minus face (the same operands, so the
+  // same value as forming it here)
+  T dM[NC], dP[NC];
+  if (o.order > 1) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) dM[q] = faceDiffM<XD, T>(n, q);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const T v = shfl(dM[q], lp);
+      dP[q] = inP ? v : T(0.0);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) dM[q] = dP[q] = T(0.0);
+  }
+  n.dM = dM;
+  n.dP = dP;
 
   // normal velocities of the minus / plus faces (PrepareRemap.cc:50-62)
   auto fnv = [&](long b) {
@@ -316,7 +335,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 
 template <class R, int M, bool PP, bool XD>
 __global__ __launch_bounds__(256) void k_sweep_wave(KPT<R> kp) {
-  sweepWaveBody<R, M, PP, XD>(kp, blockIdx.x);
+  sweepWaveBody<R, M, PP, XD>(kp, XD ? blockIdx.x : xcdBlock());
 }
 
 // all problems of an ensemble in one launch (BatchLaunchT, common.hpp)
