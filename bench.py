@@ -148,7 +148,7 @@ def bench_ensemble(args):
         c.precision = prec
     batch = mahyco_amd.Batch(cfgs)
     cells = batch.cells
-    batch.run(max(args.warmup, 6))  # the first steps also tune the launch widths
+    batch.run(max(args.warmup, 12))  # the first 10 steps also tune the launch widths
     t0 = time.perf_counter()
     batch.run(args.steps)  # ends with a stream synchronisation
     wall = time.perf_counter() - t0

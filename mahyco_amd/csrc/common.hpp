@@ -78,14 +78,33 @@ enum Mode : int {
 };
 
 // Time-loop scalars, double-buffered like the reference's n/n+1 pairs.
+// minKey / maxKey accumulate the CFL minimum and the maximum pressure of the
+// step with parity cur as order-preserving integer keys (atomic min / max,
+// exact and order-free); the finalize kernel turns them into dt and pmax
+// and resets the other parity's pair for the next step.
 struct Scalars {
   double t[2];
   double dt[2];
   double pmax;
   int err;        // MHY_ERR_* raised by a kernel (energy abort, NaN)
-  unsigned int done_blocks;
   int pad;
+  unsigned long long minKey[2], maxKey[2];
 };
+
+// Integer keys ordered like the doubles they encode.  minKey only sees
+// values >= +0 or NaN (CFL ratios), whose bit patterns already order as
+// unsigned integers, with every NaN above the initial DBL_MAX; maxKey maps
+// the sign so that all doubles order, and NaN is filtered before.
+MHY_HD unsigned long long dkeyPos(double v) { return __builtin_bit_cast(unsigned long long, v); }
+MHY_HD double dkeyPosInv(unsigned long long k) { return __builtin_bit_cast(double, k); }
+MHY_HD unsigned long long dkeySigned(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+MHY_HD double dkeySignedInv(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __builtin_bit_cast(double, b);
+}
 
 // Options and geometry constants, uniform for a launch.
 struct Opts {
@@ -132,7 +151,6 @@ struct FieldsT {
   double *UD1[3];    // nodal vector after dual sweep 1 (M ux, M uy, M)
   double *XL2;       // end-of-Lagrange node coordinate along the second sweep
   Scalars* sc;
-  double* partials;  // per-block reduction partials (2 per block)
 };
 
 // Parameters of one launch: fields, options, parity and sweep direction.
@@ -182,7 +200,8 @@ __device__ __forceinline__ unsigned xcdBlock() { return xcdBlock(blockIdx.x, gri
 
 // One launch over an ensemble of problems: the launch parameters of problem
 // p for step variant v are table[v * nprob + p]; the virtual blocks of problem
-// p are start[p] .. start[p+1]-1, laid out as rows of gx[p] blocks.
+// p are start[p] .. start[p+1]-1, laid out as rows of gx[p] blocks, and
+// owner[vb] is the problem of virtual block vb.
 template <class R>
 struct BatchLaunchT {
   const KPT<R>* table;
@@ -190,25 +209,22 @@ struct BatchLaunchT {
   const unsigned* gx;     // nprob entries
   int nprob, variant;
   unsigned total;         // start[nprob]
+  const unsigned* owner;  // total entries
 };
 using BatchLaunch = BatchLaunchT<double>;
 
-// problem p and block index inside it of virtual block vb (binary search)
+// problem p and block index inside it of virtual block vb
 template <class R>
 __device__ __forceinline__ void batchLocate(const BatchLaunchT<R>& bl, unsigned vb, int& p, unsigned& local) {
-  int lo = 0, hi = bl.nprob - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (bl.start[mid] <= vb) lo = mid;
-    else hi = mid - 1;
-  }
-  p = lo;
-  local = vb - bl.start[lo];
+  p = (int)bl.owner[vb];
+  local = vb - bl.start[p];
 }
 
 // Runs fn(kp, local block, gx, blocks of the problem) for a contiguous range
 // of virtual blocks: workgroup w takes blocks [w*per, (w+1)*per), so that it
 // stays on one problem (and its launch parameters) for as long as possible.
+// The first problem comes from the owner table (one load); the walk then
+// moves on through start[].
 template <class R, class F>
 __device__ __forceinline__ void batchForEach(const BatchLaunchT<R>& bl, F&& fn) {
   const unsigned per = (bl.total + gridDim.x - 1) / gridDim.x;

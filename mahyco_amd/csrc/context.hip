@@ -51,6 +51,9 @@ struct mhy_ctx {
   // MAHYCO_FUSE_FINAL=0 keeps it a separate kernel); finalDone records that
   // the current iteration's final remap is already done
   bool fuseFinal = true, finalDone = false;
+  // nodal remap along y: the wave kernel (default) or the per-node stencil
+  // kernel (environment MAHYCO_DUAL_Y=node)
+  bool dualWaveY = true;
   int status = MHY_OK;
   long ncell = 0, nnode = 0, nface = 0;
   std::string err;
@@ -199,27 +202,32 @@ int phaseDt(mhy_ctx* x) {
     timed(x, K_DT, [&] { launchR(x, k_dt_hold<double>, k_dt_hold<float>, dim3(1), dim3(64), kp); });
   } else {
     const unsigned nb = (unsigned)std::min<long>(kReduceBlocks, (x->ncell + kBlock - 1) / kBlock);
-    timed(x, K_DT, [&] { launchR(x, k_dt_reduce<double>, k_dt_reduce<float>, dim3(nb), dim3(kBlock), kp); });
+    timed(x, K_DT, [&] {
+      launchR(x, k_dt_reduce<double>, k_dt_reduce<float>, dim3(nb), dim3(kBlock), kp);
+      launchR(x, k_dt_finalize<double>, k_dt_finalize<float>, dim3(1), dim3(64), kp);
+    });
   }
   return checkLaunch(x, "dt");
 }
+
+using KernelFn = void (*)(KP);
+using KernelFnF = void (*)(KPT<float>);
 
 int phaseLagrange(mhy_ctx* x) {
   KP kp = makeKP(x);
   timed(x, K_NODE, [&] {
     launchR(x, k_lagrange_node<double>, k_lagrange_node<float>, gridFor(x->nnode), dim3(kBlock), kp);
   });
-  timed(x, K_CELL, [&] {
-    launchR(x, k_lagrange_cell<double>, k_lagrange_cell<float>, gridFor(x->ncell), dim3(kBlock), kp);
-  });
+  static const KernelFn kc[3] = {k_lagrange_cell<double, 1>, k_lagrange_cell<double, 2>, k_lagrange_cell<double, 3>};
+  static const KernelFnF kcf[3] = {k_lagrange_cell<float, 1>, k_lagrange_cell<float, 2>, k_lagrange_cell<float, 3>};
+  const int mi = x->o.nbmat - 1;
+  timed(x, K_CELL, [&] { launchR(x, kc[mi], kcf[mi], gridFor(x->ncell), dim3(kBlock), kp); });
   return checkLaunch(x, "lagrange");
 }
 
 // sweep kernels for the direction, the material count and the PenteBorne
 // option: the wave-shuffle kernels (default) or the per-cell stencil kernels
 // (FP64 form only)
-using KernelFn = void (*)(KP);
-using KernelFnF = void (*)(KPT<float>);
 template <class R>
 void (*sweepWaveKernel(bool xd, int m, bool pp))(KPT<R>) {
   using Fn = void (*)(KPT<R>);
@@ -267,13 +275,15 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, x->stream, kp); });
     }
   } else if (x->sweepWave && xd) {
-    // along y the per-node kernel is faster (a 2-column wave strip reads
-    // 16-byte row segments)
     const dim3 grid = dualWaveGrid(true, x->o.nx, x->o.ny);
     timed(x, K_DUAL_X, [&] { launchR(x, k_dual_wave<double, true>, k_dual_wave<float, true>, grid, dim3(kBlock), kp); });
   } else if (xd) {
     timed(x, K_DUAL_X,
           [&] { launchR(x, k_dual<double, true>, k_dual<float, true>, gridFor(x->nnode), dim3(kBlock), kp); });
+  } else if (x->sweepWave && x->dualWaveY) {
+    const dim3 grid = dualWaveGrid(false, x->o.nx, x->o.ny);
+    timed(x, K_DUAL_Y,
+          [&] { launchR(x, k_dual_wave<double, false>, k_dual_wave<float, false>, grid, dim3(kBlock), kp); });
   } else {
     timed(x, K_DUAL_Y,
           [&] { launchR(x, k_dual<double, false>, k_dual<float, false>, gridFor(x->nnode), dim3(kBlock), kp); });
@@ -411,6 +421,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   x->cfg = *cfg;
   if (const char* sk = std::getenv("MAHYCO_SWEEP_KERNEL")) x->sweepWave = (std::string(sk) != "cell");
   if (const char* ff = std::getenv("MAHYCO_FUSE_FINAL")) x->fuseFinal = (std::string(ff) != "0");
+  if (const char* dy = std::getenv("MAHYCO_DUAL_Y")) x->dualWaveY = (std::string(dy) != "node");
   const mhy_config& c = *cfg;
   auto bail = [&](int code) {
     // keep the context so the caller can read the message, unless it is
@@ -506,7 +517,6 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     if ((e = hipMalloc(&p, sizeof(Scalars))) == hipSuccess) x->allocs.push_back(p);
     f.sc = static_cast<Scalars*>(p);
   }
-  AD(f.partials, 2 * kReduceBlocks);
 #undef AR
 #undef AD
   if (e != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, std::string("device allocation: ") + hipGetErrorString(e)));
@@ -550,6 +560,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   std::memset(&s0, 0, sizeof(s0));
   s0.t[0] = 0.0;
   s0.dt[0] = h.deltat_init;
+  for (int q = 0; q < 2; ++q) s0.minKey[q] = dkeyPos(DBL_MAX), s0.maxKey[q] = dkeySigned(-DBL_MAX);
   if (hipMemcpy(f.sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(x, MHY_ERR_DEVICE, "upload scalars"));
   // computeNodeMass of the prologue (VnrRemap.cc:341)
@@ -908,6 +919,7 @@ enum BatchKernel { BK_DT = 0, BK_NODE, BK_CELL, BK_SWX, BK_SWY, BK_DUX, BK_DUY, 
 constexpr int kVariants = 16;  // cur (1 bit), sweep-1 (1), xdir (1), fuseFinal (1)
 constexpr unsigned kCand[] = {0u, 1024u, 2048u, 4096u, 8192u};  // 0: one workgroup per virtual block
 constexpr int kNCand = sizeof(kCand) / sizeof(kCand[0]);
+constexpr int kTuneSteps = 2 * kNCand;  // each candidate over two steps
 template <class R>
 void (*sweepBatchKernel(bool xd, int m, bool pp))(BatchLaunchT<R>) {
   using Fn = void (*)(BatchLaunchT<R>);
@@ -928,9 +940,12 @@ struct mhy_batch {
   int m = 0;
   bool pp = false;
   bool mixed = false;  // every problem in the mixed FP32 / FP64 form
+  bool dualWaveY = true;  // nodal remap along y with the wave kernel
   void* dTable = nullptr;  // KPT<double> or KPT<float> entries
   unsigned* dStart = nullptr;  // BK_COUNT x (n+1)
   unsigned* dGx = nullptr;     // BK_COUNT x n
+  unsigned* dOwner = nullptr;  // problem of each virtual block, kernel after kernel
+  size_t ownerOff[BK_COUNT] = {};
   unsigned total[BK_COUNT] = {};
   unsigned blocks[BK_COUNT] = {};  // workgroups per launch (0: total)
   int cur = 0, cur0 = 0;
@@ -959,21 +974,28 @@ int batchEnqueueStepR(mhy_batch* b, int cur, bool xty, bool timing) {
   const int n = (int)b->ctx.size();
   auto launch = [&](int k, void (*fn)(BatchLaunchT<R>), int variant) {
     BatchLaunchT<R> bl{static_cast<const KPT<R>*>(b->dTable), b->dStart + (size_t)k * (n + 1),
-                       b->dGx + (size_t)k * n, n, variant, b->total[k]};
+                       b->dGx + (size_t)k * n, n, variant, b->total[k], b->dOwner + b->ownerOff[k]};
     const unsigned grid = b->blocks[k] ? std::min(b->blocks[k], b->total[k]) : b->total[k];
     if (timing) (void)hipEventRecord(b->ev[2 * k], b->stream);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, b->stream, bl);
     if (timing) (void)hipEventRecord(b->ev[2 * k + 1], b->stream);
   };
   launch(BK_DT, k_dt_reduce_batch<R>, cur);
+  {
+    BatchLaunchT<R> bl{static_cast<const KPT<R>*>(b->dTable), nullptr, nullptr, n, cur, 0u, nullptr};
+    hipLaunchKernelGGL(k_dt_finalize_batch<R>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, bl);
+  }
   launch(BK_NODE, k_lagrange_node_batch<R>, cur);
-  launch(BK_CELL, k_lagrange_cell_batch<R>, cur);
+  static void (*const cellK[3])(BatchLaunchT<R>) = {k_lagrange_cell_batch<R, 1>, k_lagrange_cell_batch<R, 2>,
+                                                     k_lagrange_cell_batch<R, 3>};
+  launch(BK_CELL, cellK[b->m - 1], cur);
   for (int sweep = 1; sweep <= 2; ++sweep) {
     const bool xd = (sweep == 1) ? xty : !xty;
     const int fuse = (sweep == 2) ? 1 : 0;
     const int v = cur + 2 * (sweep - 1) + 4 * (xd ? 1 : 0);
     launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R>(xd, b->m, b->pp), v + 8 * fuse);
     if (xd) launch(BK_DUX, k_dual_wave_batch<R, true>, v);
+    else if (b->dualWaveY) launch(BK_DUY, k_dual_wave_batch<R, false>, v);
     else launch(BK_DUY, k_dual_batch<R, false>, v);
   }
   hipError_t e = hipGetLastError();
@@ -1033,6 +1055,7 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
   b->m = cfgs[0].nbmat;
   b->pp = cfgs[0].pp == 1;
   b->mixed = cfgs[0].precision == 1;
+  if (const char* dy = std::getenv("MAHYCO_DUAL_Y")) b->dualWaveY = (std::string(dy) != "node");
   if (const char* g = std::getenv("MAHYCO_BATCH_GRAPH")) b->useGraph = (std::string(g) != "0");
   if (const char* t = std::getenv("MAHYCO_BATCH_TUNE")) b->tune = (std::string(t) != "0");
   b->ctx.resize(n, nullptr);
@@ -1080,6 +1103,7 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
         case BK_SWX: g = sweepWaveGrid(true, nx, ny); break;
         case BK_SWY: g = sweepWaveGrid(false, nx, ny); break;
         case BK_DUX: g = dualWaveGrid(true, nx, ny); break;
+        case BK_DUY: g = b->dualWaveY ? dualWaveGrid(false, nx, ny) : gridFor(x->nnode); break;
         default: g = gridFor(x->nnode); break;  // per-node dual along y
       }
       start[(size_t)k * (n + 1) + i] = acc;
@@ -1089,15 +1113,23 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
     start[(size_t)k * (n + 1) + n] = acc;
     b->total[k] = acc;
   }
+  std::vector<unsigned> owner;
+  for (int k = 0; k < BK_COUNT; ++k) {
+    b->ownerOff[k] = owner.size();
+    for (int i = 0; i < n; ++i)
+      owner.insert(owner.end(), start[(size_t)k * (n + 1) + i + 1] - start[(size_t)k * (n + 1) + i], (unsigned)i);
+  }
   if (hipMalloc(&b->dTable, table.size() * sizeof(KP)) != hipSuccess ||
       hipMalloc(&b->dStart, start.size() * sizeof(unsigned)) != hipSuccess ||
-      hipMalloc(&b->dGx, gx.size() * sizeof(unsigned)) != hipSuccess)
+      hipMalloc(&b->dGx, gx.size() * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&b->dOwner, std::max<size_t>(owner.size(), 1) * sizeof(unsigned)) != hipSuccess)
     return batchFail(b, MHY_ERR_DEVICE, "batch tables");
   if (hipMemcpy(b->dTable, table.data(), table.size() * sizeof(KP), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->dStart, start.data(), start.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->dGx, gx.data(), gx.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(b->dGx, gx.data(), gx.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->dOwner, owner.data(), owner.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess)
     return batchFail(b, MHY_ERR_DEVICE, "batch tables upload");
-  if (!b->tune) b->tuneStep = kNCand;  // keep one workgroup per virtual block
+  if (!b->tune) b->tuneStep = kTuneSteps;  // keep one workgroup per virtual block
   return MHY_OK;
 }
 
@@ -1111,6 +1143,7 @@ int mhy_batch_destroy(mhy_batch* b) {
   if (b->dTable) (void)hipFree(b->dTable);
   if (b->dStart) (void)hipFree(b->dStart);
   if (b->dGx) (void)hipFree(b->dGx);
+  if (b->dOwner) (void)hipFree(b->dOwner);
   for (mhy_ctx* x : b->ctx) mhy_destroy(x);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1132,18 +1165,21 @@ int mhy_batch_run(mhy_batch* b, int nsteps) {
   if (b->ctx.empty() || !b->dTable) return MHY_ERR_ARG;
   for (int s = 0; s < nsteps; ++s) {
     int st;
-    if (b->tuneStep < kNCand) {
-      // one step with every kernel at candidate tuneStep, timed
-      for (int k = 0; k < BK_COUNT; ++k) b->blocks[k] = kCand[b->tuneStep];
+    if (b->tuneStep < kTuneSteps) {
+      // two consecutive steps (both sweep orders: a sweep kernel is sweep 1
+      // in one and the heavier fused sweep 2 in the other) with every kernel
+      // at candidate tuneStep / 2, timed
+      const int cand = b->tuneStep / 2;
+      for (int k = 0; k < BK_COUNT; ++k) b->blocks[k] = kCand[cand];
       if ((st = batchEnqueueStep(b, b->cur, b->xty, true)) != MHY_OK) return st;
       if (hipStreamSynchronize(b->stream) != hipSuccess) return batchFail(b, MHY_ERR_DEVICE, "batch sync");
       const bool verbose = std::getenv("MAHYCO_BATCH_VERBOSE") != nullptr;
       for (int k = 0; k < BK_COUNT; ++k) {  // every kernel runs once per step
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, b->ev[2 * k], b->ev[2 * k + 1]) == hipSuccess) b->tuneMs[k][b->tuneStep] = ms;
-        if (verbose) std::fprintf(stderr, "[batch tune] kernel %d blocks %u: %.4f ms\n", k, kCand[b->tuneStep], ms);
+        if (hipEventElapsedTime(&ms, b->ev[2 * k], b->ev[2 * k + 1]) == hipSuccess) b->tuneMs[k][cand] += ms;
+        if (verbose) std::fprintf(stderr, "[batch tune] kernel %d blocks %u: %.4f ms\n", k, kCand[cand], ms);
       }
-      if (++b->tuneStep == kNCand) {
+      if (++b->tuneStep == kTuneSteps) {
         batchPickBlocks(b);
         if (verbose)
           for (int k = 0; k < BK_COUNT; ++k) std::fprintf(stderr, "[batch tune] kernel %d -> %u\n", k, b->blocks[k]);

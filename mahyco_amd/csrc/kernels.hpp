@@ -10,16 +10,20 @@ namespace mhy {
 template <class R>
 __global__ void k_dt_reduce(KPT<R> kp);
 template <class R>
+__global__ void k_dt_finalize(KPT<R> kp);
+template <class R>
+__global__ void k_dt_finalize_batch(BatchLaunchT<R> bl);
+template <class R>
 __global__ void k_dt_hold(KPT<R> kp);
 template <class R>
 __global__ void k_lagrange_node(KPT<R> kp);
-template <class R>
+template <class R, int M>
 __global__ void k_lagrange_cell(KPT<R> kp);
 template <class R>
 __global__ void k_dt_reduce_batch(BatchLaunchT<R> bl);
 template <class R>
 __global__ void k_lagrange_node_batch(BatchLaunchT<R> bl);
-template <class R>
+template <class R, int M>
 __global__ void k_lagrange_cell_batch(BatchLaunchT<R> bl);
 template <class R, int M, bool PP, bool XD>
 __global__ void k_sweep_wave_batch(BatchLaunchT<R> bl);

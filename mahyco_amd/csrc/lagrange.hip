@@ -23,11 +23,6 @@ __device__ __forceinline__ double waveMax(double v) {
   }
   return v;
 }
-__device__ __forceinline__ double loadCoherent(const double* p) {
-  unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)b);
-}
 
 __device__ __forceinline__ void raise(Scalars* sc, int code) { atomicCAS(&sc->err, 0, code); }
 
@@ -43,10 +38,11 @@ __device__ __forceinline__ D2 nodeX(const KPT<R>& kp, long j, long i) {
 
 }  // namespace
 
-// vnr_remap/VnrRemap.cc:14-50 (computeDeltaT + computeTime) fused with the
-// max-pressure reduction of PhaseLagrange.cc:55-61.  Min and max ignore NaN
-// values like the reference's reducers; the last block to finish combines
-// the per-block partials.
+// vnr_remap/VnrRemap.cc:14-50 (computeDeltaT) fused with the max-pressure
+// reduction of PhaseLagrange.cc:55-61.  Min and max ignore NaN values like
+// the reference's reducers.  Each block reduces its cells and adds them to
+// the step's accumulators with one 64-bit atomic min / max (exact and
+// order-free, no fence); k_dt_finalize then forms dt and t (computeTime).
 // b: block index among the nb blocks working on this problem.  The
 // reduction is FP64 in both forms (FP32 inputs are widened).
 template <class R>
@@ -72,7 +68,6 @@ __device__ __forceinline__ void dtReduceBody(const KPT<R>& kp, unsigned b, unsig
     pm = (pc > pm) ? pc : pm;
   }
   __shared__ double smn[4], smx[4];
-  __shared__ bool last;
   vmin = waveMin(vmin);
   pm = waveMax(pm);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -81,40 +76,33 @@ __device__ __forceinline__ void dtReduceBody(const KPT<R>& kp, unsigned b, unsig
   if (threadIdx.x == 0) {
     double lo = smn[0], hi = smx[0];
     for (int k = 1; k < (int)(blockDim.x >> 6); ++k) lo = smin(lo, smn[k]), hi = (smx[k] > hi) ? smx[k] : hi;
-    f.partials[2 * b] = lo;
-    f.partials[2 * b + 1] = hi;
-    __threadfence();
-    unsigned int done = atomicAdd(&f.sc->done_blocks, 1u);
-    last = (done == nb - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  double lo = DBL_MAX, hi = -DBL_MAX;
-  for (int k = threadIdx.x; k < (int)nb; k += blockDim.x) {
-    lo = smin(lo, loadCoherent(&f.partials[2 * k]));
-    double pb = loadCoherent(&f.partials[2 * k + 1]);
-    hi = (pb > hi) ? pb : hi;
-  }
-  lo = waveMin(lo);
-  hi = waveMax(hi);
-  if (l == 0) smn[w] = lo, smx[w] = hi;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    lo = smn[0], hi = smx[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) lo = smin(lo, smn[k]), hi = (smx[k] > hi) ? smx[k] : hi;
     Scalars* sc = f.sc;
-    const double dtn = sc->dt[kp.cur];
-    const double dt1 = smin(lo, 1.05 * dtn);
-    sc->dt[kp.nxt] = dt1;
-    sc->t[kp.nxt] = sc->t[kp.cur] + dt1;
-    sc->pmax = hi;
-    sc->done_blocks = 0;
+    atomicMin(&sc->minKey[kp.cur], dkeyPos(lo));
+    atomicMax(&sc->maxKey[kp.cur], dkeySigned(hi));
   }
 }
 
 template <class R>
 __global__ __launch_bounds__(256) void k_dt_reduce(KPT<R> kp) {
   dtReduceBody(kp, blockIdx.x, gridDim.x);
+}
+
+// dt_{n+1} = min(CFL minimum, 1.05 dt_n) and t_{n+1} (VnrRemap.cc:45-50,
+// computeTime); resets the accumulators of the next step's parity.
+__device__ __forceinline__ void dtFinalize(Scalars* sc, int cur, int nxt) {
+  const double lo = dkeyPosInv(sc->minKey[cur]);
+  const double dtn = sc->dt[cur];
+  const double dt1 = smin(lo, 1.05 * dtn);
+  sc->dt[nxt] = dt1;
+  sc->t[nxt] = sc->t[cur] + dt1;
+  sc->pmax = dkeySignedInv(sc->maxKey[cur]);
+  sc->minKey[nxt] = dkeyPos(DBL_MAX);
+  sc->maxKey[nxt] = dkeySigned(-DBL_MAX);
+}
+
+template <class R>
+__global__ void k_dt_finalize(KPT<R> kp) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dtFinalize(kp.f.sc, kp.cur, kp.nxt);
 }
 
 // sansLagrange: deltat_nplus1 = deltat_n (VnrRemap.cc:143-146)
@@ -236,18 +224,32 @@ __global__ __launch_bounds__(256) void k_lagrange_node(KPT<R> kp) {
 // mu_k V rho in exact arithmetic), so that mass stays conserved to FP64
 // round-off whatever the FP32 density storage (periodic cells, whose
 // density is reset to its initial value, keep mu_k V rho).
-template <class R>
+template <class R, int M>
 __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
   const Opts& o = kp.o;
   const auto& f = kp.f;
   const long c = (long)b * blockDim.x + threadIdx.x;
   if (c >= kp.ncell) return;
-  const int m = o.nbmat;
+  constexpr int m = M;  // material count (template: the loops unroll)
   const long nx = o.nx;
   const long j = c / nx, i = c - j * nx;
   const int cur = kp.cur, nxt = kp.nxt;
   const Scalars* sc = f.sc;
   const double dt1 = sc->dt[nxt];
+  // the cell's step-start words, loaded up front: their uses below are
+  // conditional, and unconditional loads can all be in flight together
+  double fk[NMAT], Mk0[NMAT], rk0[NMAT], qk0[NMAT], ek0[NMAT], pk0[NMAT], mu0[NMAT];
+  for (int k = 0; k < m; ++k) {
+    fk[k] = f.f_k[k][c];
+    Mk0[k] = f.M_k[k][c];
+    rk0[k] = (double)f.rho_k[cur][k][c];
+    qk0[k] = (double)f.q_k[cur][k][c];
+    ek0[k] = (double)f.e_k[cur][k][c];
+    pk0[k] = (double)f.p_k[cur][k][c];
+    mu0[k] = (double)f.mu_k[k][c];
+  }
+  const double rhoS = (double)f.rho[cur][c], divuS = (double)f.divu[cur][c], cS = (double)f.c[cur][c];
+  const double vol0 = f.vol[cur][c], Mc0 = f.M[c];
   // end-of-Lagrange node positions, node order BL, BR, TR, TL
   D2 X[4];
   {
@@ -268,14 +270,13 @@ __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
   double V = 0.0;
   for (int s = 0; s < 4; ++s) V = V + subVolume(xc, X[(s + 3) & 3], X[s], X[(s + 1) & 3]);
 
-  double fk[NMAT], rk1[NMAT];
+  double rk1[NMAT];
   double rho1 = 0.0;
   for (int k = 0; k < m; ++k) {
-    fk[k] = f.f_k[k][c];
     // absent materials keep the stale buffer value in the reference; their
     // volume fraction is exactly zero, so the value never contributes
-    rk1[k] = (fk[k] > o.threshold) ? f.M_k[k][c] / (fk[k] * V)
-                                   : (o.mode == MODE_FULL ? 0.0 : f.rho_k[nxt][k][c]);
+    rk1[k] = (fk[k] > o.threshold) ? Mk0[k] / (fk[k] * V)
+                                   : (o.mode == MODE_FULL ? 0.0 : (double)f.rho_k[nxt][k][c]);
     rho1 += fk[k] * rk1[k];
   }
   const bool periodicCell = o.periodic_right && (i == 0 || i == nx - 1);
@@ -290,13 +291,12 @@ __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
     // mixed form: the step-start densities from the FP64 masses and volume,
     // so that 1/rho1 - 1/rho0 (divided by dt in div u) carries no FP32
     // rounding of the stored density
-    const double vol0 = f.vol[cur][c];
-    const double rho0 = kMixed<R> ? f.M[c] / vol0 : (double)f.rho[cur][c];
+    const double rho0 = kMixed<R> ? Mc0 / vol0 : rhoS;
     const double tau1 = 0.5 * (1.0 / rho1 + 1.0 / rho0);
     const double div1 = 1.0 / dt1 * (1.0 / rho1 - 1.0 / rho0) / tau1;
-    if (f.divu[cur][c] < 0.0) {
+    if (divuS < 0.0) {
       const double r0 = vol0;
-      q1 = 1.0 / tau1 * (-0.5 * sqrt(r0) * (double)f.c[cur][c] * div1 + (o.gamma + 1) / 2.0 * r0 * div1 * div1);
+      q1 = 1.0 / tau1 * (-0.5 * sqrt(r0) * cS * div1 + (o.gamma + 1) / 2.0 * r0 * div1 * div1);
     } else {
       q1 = 0.0;
     }
@@ -308,16 +308,16 @@ __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
     for (int k = 0; k < m; ++k) {
       ek1[k] = 0.0;
       const double r1 = rk1[k];
-      const double r0 = (kMixed<R> && fk[k] > o.threshold) ? f.M_k[k][c] / (fk[k] * vol0) : (double)f.rho_k[cur][k][c];
+      const double r0 = (kMixed<R> && fk[k] > o.threshold) ? Mk0[k] / (fk[k] * vol0) : rk0[k];
       if ((r1 > o.threshold) && (r0 > o.threshold)) {
-        const double qn = (double)f.q_k[cur][k][c];
+        const double qn = qk0[k];
         double pseudo = 0.0;
         if ((o.pseudo_centree == 1) && ((qk1[k] + qn) * (1.0 / r1 - 1.0 / r0) > 0.)) pseudo = 0.5 * (qk1[k] + qn);
         if (o.pseudo_centree == 0) pseudo = qk1[k];
         const double den = 1 + 0.5 * (o.gammap[k] - 1.0) * r1 * (1.0 / r1 - 1.0 / r0);
-        const double num = (double)f.e_k[cur][k][c] - (0.5 * (double)f.p_k[cur][k][c] + pseudo) * (1.0 / r1 - 1.0 / r0);
+        const double num = ek0[k] - (0.5 * pk0[k] + pseudo) * (1.0 / r1 - 1.0 / r0);
         ek1[k] = num / den;
-        e1 += (double)f.mu_k[k][c] * ek1[k];
+        e1 += mu0[k] * ek1[k];
         if (fabs(ek1[k]) > 5. && o.abort_energy) bad = true;
       }
     }
@@ -329,9 +329,9 @@ __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
         pk1[k] = (g - 1.0) * rk1[k] * ek1[k];
         ck1[k] = sqrt(g * (g - 1.0) * ek1[k]);
       } else if (o.eos[k] == MHY_EOS_VOID) {
-        pk1[k] = f.p_k[cur][k][c];
+        pk1[k] = pk0[k];
         ck1[k] = f.c_k[cur][k][c];
-        ek1[k] = f.e_k[cur][k][c];
+        ek1[k] = ek0[k];
         e1 = 0.0;
       } else {
         pk1[k] = f.p_k[nxt][k][c];
@@ -348,7 +348,7 @@ __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
     }
     if (rho1 > 0.) c1 = sqrt(o.gammap[0] * (o.gammap[0] - 1.0) * e1);
     for (int k = 0; k < m; ++k)
-      if (o.eos[k] == MHY_EOS_VOID) e1 += (double)f.mu_k[k][c] * ek1[k];
+      if (o.eos[k] == MHY_EOS_VOID) e1 += mu0[k] * ek1[k];
     f.divu[nxt][c] = div1;
     for (int k = 0; k < m; ++k) f.q_k[nxt][k][c] = qk1[k];
     if (o.mode == MODE_LAGRANGE) {
@@ -377,11 +377,11 @@ __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
   for (int k = 0; k < m; ++k) {
     f.UL[k][c] = fk[k] * V;
     if (kMixed<R> && !periodicCell) {
-      const double Mk = f.M_k[k][c];
+      const double Mk = Mk0[k];
       f.UL[m + k][c] = Mk;
       f.UL[2 * m + k][c] = Mk * e1;
     } else {
-      const double mu = f.mu_k[k][c];
+      const double mu = mu0[k];
       f.UL[m + k][c] = mu * V * rho1;
       f.UL[2 * m + k][c] = mu * V * rho1 * e1;
     }
@@ -401,9 +401,9 @@ __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
   f.YcL[c] = r.y;
 }
 
-template <class R>
+template <class R, int M>
 __global__ __launch_bounds__(256) void k_lagrange_cell(KPT<R> kp) {
-  lagrangeCellBody(kp, blockIdx.x);
+  lagrangeCellBody<R, M>(kp, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- batches
@@ -418,23 +418,37 @@ __global__ __launch_bounds__(256) void k_dt_reduce_batch(BatchLaunchT<R> bl) {
     __syncthreads();  // the shared scratch is reused by the next virtual block
   });
 }
+// one thread per problem
+template <class R>
+__global__ __launch_bounds__(256) void k_dt_finalize_batch(BatchLaunchT<R> bl) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= bl.nprob) return;
+  const KPT<R>& kp = bl.table[bl.variant * bl.nprob + p];
+  dtFinalize(kp.f.sc, kp.cur, kp.nxt);
+}
 template <class R>
 __global__ __launch_bounds__(256) void k_lagrange_node_batch(BatchLaunchT<R> bl) {
   batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeNodeBody(kp, b); });
 }
-template <class R>
+template <class R, int M>
 __global__ __launch_bounds__(256) void k_lagrange_cell_batch(BatchLaunchT<R> bl) {
-  batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeCellBody(kp, b); });
+  batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeCellBody<R, M>(kp, b); });
 }
 
 #define MHY_LAGRANGE_INST(R)                                                 \
   template __global__ void k_dt_reduce<R>(KPT<R>);                           \
+  template __global__ void k_dt_finalize<R>(KPT<R>);                         \
+  template __global__ void k_dt_finalize_batch<R>(BatchLaunchT<R>);          \
   template __global__ void k_dt_hold<R>(KPT<R>);                             \
   template __global__ void k_lagrange_node<R>(KPT<R>);                       \
-  template __global__ void k_lagrange_cell<R>(KPT<R>);                       \
+  template __global__ void k_lagrange_cell<R, 1>(KPT<R>);                    \
+  template __global__ void k_lagrange_cell<R, 2>(KPT<R>);                    \
+  template __global__ void k_lagrange_cell<R, 3>(KPT<R>);                    \
   template __global__ void k_dt_reduce_batch<R>(BatchLaunchT<R>);            \
   template __global__ void k_lagrange_node_batch<R>(BatchLaunchT<R>);        \
-  template __global__ void k_lagrange_cell_batch<R>(BatchLaunchT<R>);
+  template __global__ void k_lagrange_cell_batch<R, 1>(BatchLaunchT<R>);     \
+  template __global__ void k_lagrange_cell_batch<R, 2>(BatchLaunchT<R>);     \
+  template __global__ void k_lagrange_cell_batch<R, 3>(BatchLaunchT<R>);
 MHY_LAGRANGE_INST(double)
 MHY_LAGRANGE_INST(float)
 #undef MHY_LAGRANGE_INST

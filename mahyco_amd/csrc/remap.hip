@@ -5,62 +5,17 @@
 // Layout: structure-of-arrays in HBM (bulk fields of storage type R: FP64 or
 // FP32; masses, fluxes and the nodal momentum FP64), one thread per node
 // (cell) with the stencil along the sweep read through the L1/L2 caches.
-// Arithmetic is FP64 in both forms.
+// Arithmetic is FP64, except the velocity / gradient arithmetic of the wave
+// kernels in the mixed form (FP32).  The slope limiters are those of the
+// cell sweeps (sweep_math.hpp).
+#include <type_traits>
+
 #include "common.hpp"
 #include "final_math.hpp"
 #include "kernels.hpp"
+#include "sweep_math.hpp"
 
 namespace mhy {
-
-namespace {
-
-// remap/UtilesRemap.cc:9-23
-__device__ __forceinline__ double fluxLimiter(int lim, double r) {
-  if (lim == MHY_LIM_MINMOD) return smax(0.0, smin(1.0, r));
-  if (lim == MHY_LIM_SUPERBEE) return smax(0.0, smax(smin(2.0 * r, 1.0), smin(r, 2.0)));
-  if (lim == MHY_LIM_VANLEER) return (r <= 0.0) ? 0.0 : 2.0 * r / (1.0 + r);
-  return 0.0;
-}
-
-// remap/UtilesRemap.cc:25-72
-__device__ double fluxLimiterPP(int lim, double gradplus, double gradmoins, double yplus, double ymoins, double h0,
-                                double hplus, double hmoins) {
-  double grady = 0.0;
-  if (gradplus * gradmoins < 0.0) return 0.;
-  if (lim == MHY_LIM_MINMODG) {
-    grady = ((yplus - ymoins) > 0.) ? smin(fabs(gradplus), fabs(gradmoins)) : -smin(fabs(gradplus), fabs(gradmoins));
-  } else if (lim == MHY_LIM_SUPERBEEG) {
-    grady = ((yplus - ymoins) > 0.) ? smax(fabs(gradplus), fabs(gradmoins)) : -smax(fabs(gradplus), fabs(gradmoins));
-  } else if (lim == MHY_LIM_VANLEERG) {
-    double lp = (h0 / 2. + hplus) / (h0 + hplus + hmoins);
-    double lm = (h0 / 2. + hmoins) / (h0 + hplus + hmoins);
-    if ((lp * gradplus + lm * gradmoins) != 0.) grady = gradplus * gradmoins / (lp * gradplus + lm * gradmoins);
-    else grady = 0.;
-  } else if (lim == MHY_LIM_ULTRABEEG) {
-    grady = (yplus - ymoins) / h0;
-  } else if (lim == MHY_LIM_ARITHMETICG) {
-    double lp = (h0 / 2. + hplus) / (h0 + hplus + hmoins);
-    double lm = (h0 / 2. + hmoins) / (h0 + hplus + hmoins);
-    grady = lm * gradplus + lp * gradmoins;
-  }
-  double gMp = gradplus * (h0 + hplus) / h0;
-  double gMm = gradmoins * (h0 + hmoins) / h0;
-  double gM = smin(fabs(gMp), fabs(gMm));
-  if ((yplus - ymoins) > 0.) grady = smin(fabs(gM), fabs(grady));
-  else grady = -smin(fabs(gM), fabs(grady));
-  return grady;
-}
-
-// remap/UtilesRemap-Impl.h:99-126, one component
-__device__ __forceinline__ double limitedGradient(int lim, double gp, double gm, double phi, double phip,
-                                                  double phim, double h0, double hp, double hm) {
-  (void)phi;
-  if (lim < MHY_LIM_MINMODG)
-    return (fluxLimiter(lim, divideNoExcept(gp, gm)) * gm + fluxLimiter(lim, divideNoExcept(gm, gp)) * gp) / 2.0;
-  return fluxLimiterPP(lim, gp, gm, phip, phim, h0, hp, hm);
-}
-
-}  // namespace
 
 // Nodal momentum remap of one sweep (remap/DualPhaseRemap1.cc:18-136,
 // remap/DualPhaseRemap2.cc:18-127 with remap/UtilesDualRemap.cc and
@@ -137,8 +92,8 @@ __device__ __forceinline__ void dualBody(const KPT<R>& kp, unsigned b) {
       hm = 0.5 * (xl[d] - xl[d - 2]);
       hp = 0.5 * (xl[d + 2] - xl[d]);
     }
-    g[0] = limitedGradient(o.limiter, gr0, gl0, dpx[d], dpx[d + 1], dpx[d - 1], h0, hp, hm);
-    g[1] = limitedGradient(o.limiter, gr1, gl1, dpy[d], dpy[d + 1], dpy[d - 1], h0, hp, hm);
+    g[0] = sw::limitedGradient<double>(o.limiter, gr0, gl0, dpx[d + 1], dpx[d - 1], h0, hp, hm);
+    g[1] = sw::limitedGradient<double>(o.limiter, gr1, gl1, dpy[d + 1], dpy[d - 1], h0, hp, hm);
   };
   double gM[2], g0[2], gP[2];
   gradAt(3, g0);
@@ -238,23 +193,28 @@ __global__ __launch_bounds__(256) void k_dual_batch(BatchLaunchT<R> bl) {
 
 // The same nodal remap with one lane per node and the neighbour values taken
 // by wave shuffles.  A wave covers a strip of nodes along the sweep: along x,
-// 64 nodes of a node row; along y, 2 node columns x 32 node rows.  A lane
+// 64 nodes of a node row; along y, 4 node columns x 16 node rows.  A lane
 // needs its neighbours' gradients, which need nodes two further away, so the
-// 3 strip nodes at each end are halo (58 of 64, resp. 26 of 32 nodes are
+// 3 strip nodes at each end are halo (58 of 64, resp. 10 of 16 nodes are
 // updated).  Each lane computes its own DualPhi, Lagrange coordinate and
-// limited gradient once.
+// limited gradient once (the per-node kernel above forms three gradients per
+// node).
 namespace {
 constexpr int kDW = 64;
-constexpr int kDYCols = 2;
+constexpr int kDYCols = 4;
 constexpr int kDYRows = kDW / kDYCols;
 constexpr int kDXOut = kDW - 6;
 constexpr int kDYOut = kDYRows - 6;
 __device__ __forceinline__ double dshfl(double v, int src) { return __shfl(v, src, kDW); }
+__device__ __forceinline__ float dshfl(float v, int src) { return __shfl(v, src, kDW); }
 }  // namespace
 
 // b: linear block index (strips numbered row-major along x, as the sweeps)
 template <class R, bool XD>
 __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
+  // velocity / gradient arithmetic: FP32 in the mixed form; masses, momenta
+  // and coordinates stay FP64
+  using T = typename std::conditional<kMixed<R>, float, double>::type;
   constexpr int S = XD ? 1 : kDYCols;  // lane distance of the along-sweep neighbour
   const Opts& o = kp.o;
   const auto& f = kp.f;
@@ -286,12 +246,14 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   const double dt1 = f.sc->dt[kp.nxt];
   const int order2 = o.order - 1;
 
-  // own DualPhi and Lagrange coordinate
-  double dpx0 = 0.0, dpy0 = 0.0, xl0 = 0.0;
+  // own DualPhi and Lagrange coordinate (coordinates stay FP64: only their
+  // differences enter T arithmetic)
+  T dpx0 = T(0.0), dpy0 = T(0.0);
+  double xl0 = 0.0;
   if (in) {
     if (kp.sweep == 1) {
       const double ux1 = (double)f.ux[kp.nxt][p], uy1 = (double)f.uy[kp.nxt][p];
-      dpx0 = ux1, dpy0 = uy1;
+      dpx0 = T(ux1), dpy0 = T(uy1);
       const D2 e = eulerNode(o, j, i);
       xl0 = XD ? e.x + ux1 * dt1 : e.y + uy1 * dt1;
       // Lagrange coordinate along the second sweep's axis (the second sweep
@@ -299,12 +261,13 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
       if (output) f.XL2[p] = XD ? e.y + uy1 * dt1 : e.x + ux1 * dt1;
     } else {
       const double M1 = f.UD1[2][p];
-      dpx0 = f.UD1[0][p] / M1, dpy0 = f.UD1[1][p] / M1;
+      dpx0 = T(f.UD1[0][p] / M1), dpy0 = T(f.UD1[1][p] / M1);
       xl0 = f.XL2[p];
     }
   }
   // nodes a-2 .. a+2 (slots 1..5 of the reference stencil a-3 .. a+3)
-  double dpx[7], dpy[7], xl[7];
+  T dpx[7], dpy[7];
+  double xl[7];
   bool inS[7];
 #pragma unroll
   for (int d = 1; d <= 5; ++d) {
@@ -314,47 +277,48 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
     dpx[d] = dshfl(dpx0, srcC), dpy[d] = dshfl(dpy0, srcC), xl[d] = dshfl(xl0, srcC);
     const long b = a + off;
     inS[d] = (b >= 0 && b <= na);
-    if (!inS[d]) dpx[d] = dpy[d] = xl[d] = 0.0;
+    if (!inS[d]) dpx[d] = dpy[d] = T(0.0), xl[d] = 0.0;
   }
   dpx[3] = dpx0, dpy[3] = dpy0, xl[3] = xl0;
   inS[0] = (a - 3 >= 0), inS[6] = (a + 3 <= na);
-  dpx[0] = dpy[0] = xl[0] = dpx[6] = dpy[6] = xl[6] = 0.0;
+  dpx[0] = dpy[0] = dpx[6] = dpy[6] = T(0.0);
+  xl[0] = xl[6] = 0.0;
 
   // own limited nodal gradient (UtilesRemap-Impl.h:8-97).  Each lane forms
   // the divided differences towards its minus neighbour; those towards the
   // plus neighbour are the next lane's (the same operands).
   const int lmS = (lane - S >= 0) ? lane - S : lane, lpS = (lane + S < kDW) ? lane + S : lane;
-  double gl0 = 0.0, gl1 = 0.0;
+  T gl0 = T(0.0), gl1 = T(0.0);
   if (o.order > 1) {
-    gl0 = (dpx[3] - dpx[2]) / (xl[3] - xl[2]);
-    gl1 = (dpy[3] - dpy[2]) / (xl[3] - xl[2]);
+    gl0 = (dpx[3] - dpx[2]) / T(xl[3] - xl[2]);
+    gl1 = (dpy[3] - dpy[2]) / T(xl[3] - xl[2]);
   }
-  const double gr0 = dshfl(gl0, lpS), gr1 = dshfl(gl1, lpS);
-  double g0[2] = {0.0, 0.0};
+  const T gr0 = dshfl(gl0, lpS), gr1 = dshfl(gl1, lpS);
+  T g0[2] = {T(0.0), T(0.0)};
   if (in && o.order > 1 && inS[2] && inS[4]) {
     const int d = 3;
-    const double h0 = 0.5 * (xl[d + 1] - xl[d - 1]);
-    double hm, hp;
+    const T h0 = T(0.5 * (xl[d + 1] - xl[d - 1]));
+    T hm, hp;
     if (!inS[d - 2]) {
-      hm = 0., hp = 0.5 * (xl[d + 2] - xl[d]);
+      hm = T(0.), hp = T(0.5 * (xl[d + 2] - xl[d]));
     } else if (!inS[d + 2]) {
-      hp = 0., hm = 0.5 * (xl[d] - xl[d - 2]);
+      hp = T(0.), hm = T(0.5 * (xl[d] - xl[d - 2]));
     } else {
-      hm = 0.5 * (xl[d] - xl[d - 2]);
-      hp = 0.5 * (xl[d + 2] - xl[d]);
+      hm = T(0.5 * (xl[d] - xl[d - 2]));
+      hp = T(0.5 * (xl[d + 2] - xl[d]));
     }
-    g0[0] = limitedGradient(o.limiter, gr0, gl0, dpx[d], dpx[d + 1], dpx[d - 1], h0, hp, hm);
-    g0[1] = limitedGradient(o.limiter, gr1, gl1, dpy[d], dpy[d + 1], dpy[d - 1], h0, hp, hm);
+    g0[0] = sw::limitedGradient<T>(o.limiter, gr0, gl0, dpx[d + 1], dpx[d - 1], h0, hp, hm);
+    g0[1] = sw::limitedGradient<T>(o.limiter, gr1, gl1, dpy[d + 1], dpy[d - 1], h0, hp, hm);
   }
-  double gM[2], gP[2];
+  T gM[2], gP[2];
   gM[0] = dshfl(g0[0], lmS), gM[1] = dshfl(g0[1], lmS);
   gP[0] = dshfl(g0[0], lpS), gP[1] = dshfl(g0[1], lpS);
   if (!output) return;
   // a neighbour outside the domain has a zero gradient
-  if (!inS[2]) gM[0] = gM[1] = 0.0;
-  if (!inS[4]) gP[0] = gP[1] = 0.0;
+  if (!inS[2]) gM[0] = gM[1] = T(0.0);
+  if (!inS[4]) gP[0] = gP[1] = T(0.0);
 
-  // nodal mass fluxes through the dual faces on the minus / plus side
+  // nodal mass fluxes through the dual faces on the minus / plus side (FP64)
   double plus = 0., minus = 0.;
   {
     const bool interior = (a > 0 && a < na);
@@ -396,18 +360,19 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   }
   // upwind velocities (UtilesDualRemap.cc:234-397); a missing neighbour
   // reads as 0 and its mass flux is 0
-  auto upwind = [&](bool minusSide, double flux, const double* gNb, double* r) {
+  const T dtnT = T(dtn);
+  auto upwind = [&](bool minusSide, double flux, const T* gNb, double* r) {
     const int dn = minusSide ? 2 : 4;
-    const double dNb[2] = {dpx[dn], dpy[dn]}, d0[2] = {dpx[3], dpy[3]};
-    const double uN = XD ? dpx[dn] : dpy[dn], u0 = XD ? dpx[3] : dpy[3];
-    const double u = minusSide ? 0.5 * (uN + u0) : 0.5 * (u0 + uN);
-    const int sign = (u > 0) ? 1 : -1;
-    const double h = minusSide ? (xl[3] - xl[dn]) : (xl[dn] - xl[3]);
+    const T dNb[2] = {dpx[dn], dpy[dn]}, d0[2] = {dpx[3], dpy[3]};
+    const T uN = XD ? dpx[dn] : dpy[dn], u0 = XD ? dpx[3] : dpy[3];
+    const T u = minusSide ? T(0.5) * (uN + u0) : T(0.5) * (u0 + uN);
+    const int sign = (u > T(0)) ? 1 : -1;
+    const T h = T(minusSide ? (xl[3] - xl[dn]) : (xl[dn] - xl[3]));
     const bool fromNb = minusSide ? (flux > 0) : (flux < 0);
     for (int d = 0; d < 2; ++d) {
-      const double base = fromNb ? dNb[d] : d0[d];
-      const double g = fromNb ? gNb[d] : g0[d];
-      r[d] = base + order2 * (0.5 * g * (sign * h - dtn * u));
+      const T base = fromNb ? dNb[d] : d0[d];
+      const T g = fromNb ? gNb[d] : g0[d];
+      r[d] = (double)(base + T(order2) * (T(0.5) * g * (T(sign) * h - dtnT * u)));
     }
   };
   double uvm[2], uvp[2];

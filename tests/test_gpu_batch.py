@@ -44,7 +44,7 @@ def test_batch_matches_single_problems(graphs):
     try:
         log(f"[batch] {len(cfgs)} problems, {batch.cells} cells, graphs={graphs}")
         done = 0
-        for chunk in (3, 4, 2, 3):  # 5 tuning steps, then replays
+        for chunk in (3, 4, 5, 3):  # 10 tuning steps, then replays
             batch.run(chunk)
             for s in singles:
                 s.step(chunk)

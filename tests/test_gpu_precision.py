@@ -125,7 +125,7 @@ def test_mixed_batch_matches_single_problems(graphs):
     singles = [mahyco_amd.Context(c) for c in cfgs]
     try:
         log(f"[mixed batch] {len(cfgs)} problems, graphs={graphs}")
-        for chunk in (5, 3, 3):
+        for chunk in (6, 6, 3):  # 10 tuning steps, then replays
             batch.run(chunk)
             for s in singles:
                 s.step(chunk)
