@@ -12,9 +12,11 @@ namespace mhy {
 __device__ __forceinline__ void raiseErr(Scalars* sc, int code) { atomicCAS(&sc->err, 0, code); }
 
 // U: the 3m+1 live components of the remapped vector U2 of cell c.
-// Arithmetic is FP64 in both forms: the volume fractions near 0 and 1, their
-// threshold tests and the masses stay FP64; R only sets the storage of the
-// bulk outputs.
+// The volume fractions near 0 and 1, their threshold tests, the densities,
+// specific energies and masses are FP64 in both forms (a nearly emptied
+// material's mass and energy content can lie below the FP32 range).  The
+// mixed form (R = float) takes the EOS sound speeds' square roots in FP32,
+// the precision they are stored in.
 template <class R, int M>
 __device__ __forceinline__ void finalCell(const KPT<R>& kp, long c, const double* U) {
   const Opts& o = kp.o;
@@ -72,7 +74,8 @@ __device__ __forceinline__ void finalCell(const KPT<R>& kp, long c, const double
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     pe[k] = 0.0;
-    if ((fr[k] > thr) && (U[M + k] != 0.)) pe[k] = U[2 * M + k] / U[M + k];
+    if ((fr[k] > thr) && (U[M + k] != 0.))
+      pe[k] = U[2 * M + k] / U[M + k];
   }
   const double rho1 = dn;
   double e1 = 0.;
@@ -99,7 +102,7 @@ __device__ __forceinline__ void finalCell(const KPT<R>& kp, long c, const double
     const double g = o.gammap[k];
     if (o.eos[k] == MHY_EOS_PERFECT_GAS) {
       pk[k] = (g - 1.0) * rk[k] * ek[k];
-      ck[k] = sqrt(g * (g - 1.0) * ek[k]);
+      ck[k] = kMixed<R> ? (double)sqrtf(float(g * (g - 1.0) * ek[k])) : sqrt(g * (g - 1.0) * ek[k]);
     } else if (o.eos[k] == MHY_EOS_VOID) {
       pk[k] = (double)f.p_k[cur][k][c];
       ck[k] = (double)f.c_k[cur][k][c];
@@ -119,7 +122,10 @@ __device__ __forceinline__ void finalCell(const KPT<R>& kp, long c, const double
     p1 += fr[k] * pk[k];
     if (needStale) c1 = smax(c1, ck[k]);
   }
-  if (rho1 > 0.) c1 = sqrt(o.gammap[0] * (o.gammap[0] - 1.0) * e1);
+  if (rho1 > 0.) {
+    const double c2 = o.gammap[0] * (o.gammap[0] - 1.0) * e1;
+    c1 = kMixed<R> ? (double)sqrtf(float(c2)) : sqrt(c2);
+  }
 #pragma unroll
   for (int k = 0; k < M; ++k)
     if (o.eos[k] == MHY_EOS_VOID) e1 += mu[k] * ek[k];

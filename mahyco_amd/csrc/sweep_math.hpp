@@ -158,7 +158,8 @@ __device__ __forceinline__ T sweptIntegral(const Profile<T>& pr, T h0, T ymoins,
 
 // Phi of a cell at the start of the sweep: U/V, or the PenteBorne form
 // (fractions, densities, specific energies, Lagrange-phase pseudo), formed in
-// FP64 from the FP64 remap vector and then held as T.
+// FP64 from the FP64 remap vector and then held as T; an FP32 U/V (mixed
+// form, limited slopes) is formed in FP32.
 template <int M, bool PP, class T>
 __device__ __forceinline__ void cellPhi(const double* u, double V, double qL, T* phi) {
   constexpr int NC = 3 * M + 1;
@@ -174,6 +175,11 @@ __device__ __forceinline__ void cellPhi(const double* u, double V, double qL, T*
 #pragma unroll
     for (int k = 0; k < M; ++k) phi[2 * M + k] = T((u[M + k] != 0.) ? u[2 * M + k] / u[M + k] : 0.0);
     phi[3 * M] = T(qL);
+  } else if (sizeof(T) == 4) {
+    // FP32 slopes: Phi = U * (1/V) in FP32 (one reciprocal per cell)
+    const T rV = T(1.0) / T(V);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) phi[q] = T(u[q]) * rV;
   } else {
 #pragma unroll
     for (int q = 0; q < NC; ++q) phi[q] = T(u[q] / V);
