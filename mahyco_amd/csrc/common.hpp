@@ -245,6 +245,22 @@ __device__ __forceinline__ void batchForEach(const BatchLaunchT<R>& bl, F&& fn) 
   }
 }
 
+// One workgroup per virtual block (ONE, a launch of start[nprob] workgroups)
+// or the contiguous ranges of batchForEach.  Separate kernels: the walk's
+// state costs registers (occupancy) that a single block does not need.
+template <bool ONE, class R, class F>
+__device__ __forceinline__ void batchRun(const BatchLaunchT<R>& bl, F&& fn) {
+  if (ONE) {
+    const unsigned vb = blockIdx.x;
+    if (vb >= bl.total) return;
+    const int p = (int)bl.owner[vb];
+    const unsigned s0 = bl.start[p];
+    fn(bl.table[bl.variant * bl.nprob + p], vb - s0, bl.gx[p], bl.start[p + 1] - s0);
+  } else {
+    batchForEach(bl, fn);
+  }
+}
+
 // Euler node coordinates (CartesianMesh2DGenerator.cc:52-55: x = dx*i, y = dy*j)
 MHY_HD D2 eulerNode(const Opts& o, long j, long i) { return {o.dx * (double)i, o.dy * (double)j}; }
 

@@ -920,16 +920,16 @@ constexpr int kVariants = 16;  // cur (1 bit), sweep-1 (1), xdir (1), fuseFinal 
 constexpr unsigned kCand[] = {0u, 1024u, 2048u, 4096u, 8192u};  // 0: one workgroup per virtual block
 constexpr int kNCand = sizeof(kCand) / sizeof(kCand[0]);
 constexpr int kTuneSteps = 2 * kNCand;  // each candidate over two steps
-template <class R>
+template <class R, bool ONE>
 void (*sweepBatchKernel(bool xd, int m, bool pp))(BatchLaunchT<R>) {
   using Fn = void (*)(BatchLaunchT<R>);
   static const Fn t[2][3][2] = {
-      {{k_sweep_wave_batch<R, 1, false, false>, k_sweep_wave_batch<R, 1, true, false>},
-       {k_sweep_wave_batch<R, 2, false, false>, k_sweep_wave_batch<R, 2, true, false>},
-       {k_sweep_wave_batch<R, 3, false, false>, k_sweep_wave_batch<R, 3, true, false>}},
-      {{k_sweep_wave_batch<R, 1, false, true>, k_sweep_wave_batch<R, 1, true, true>},
-       {k_sweep_wave_batch<R, 2, false, true>, k_sweep_wave_batch<R, 2, true, true>},
-       {k_sweep_wave_batch<R, 3, false, true>, k_sweep_wave_batch<R, 3, true, true>}}};
+      {{k_sweep_wave_batch<R, 1, false, false, ONE>, k_sweep_wave_batch<R, 1, true, false, ONE>},
+       {k_sweep_wave_batch<R, 2, false, false, ONE>, k_sweep_wave_batch<R, 2, true, false, ONE>},
+       {k_sweep_wave_batch<R, 3, false, false, ONE>, k_sweep_wave_batch<R, 3, true, false, ONE>}},
+      {{k_sweep_wave_batch<R, 1, false, true, ONE>, k_sweep_wave_batch<R, 1, true, true, ONE>},
+       {k_sweep_wave_batch<R, 2, false, true, ONE>, k_sweep_wave_batch<R, 2, true, true, ONE>},
+       {k_sweep_wave_batch<R, 3, false, true, ONE>, k_sweep_wave_batch<R, 3, true, true, ONE>}}};
   return t[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
 }  // namespace
@@ -972,31 +972,39 @@ int batchFail(mhy_batch* b, int code, const std::string& msg) {
 template <class R>
 int batchEnqueueStepR(mhy_batch* b, int cur, bool xty, bool timing) {
   const int n = (int)b->ctx.size();
-  auto launch = [&](int k, void (*fn)(BatchLaunchT<R>), int variant) {
+  // fnLoop walks contiguous virtual-block ranges (blocks[k] workgroups);
+  // fnOne runs one workgroup per virtual block (blocks[k] == 0)
+  using Fn = void (*)(BatchLaunchT<R>);
+  auto launch = [&](int k, Fn fnLoop, Fn fnOne, int variant) {
     BatchLaunchT<R> bl{static_cast<const KPT<R>*>(b->dTable), b->dStart + (size_t)k * (n + 1),
                        b->dGx + (size_t)k * n, n, variant, b->total[k], b->dOwner + b->ownerOff[k]};
-    const unsigned grid = b->blocks[k] ? std::min(b->blocks[k], b->total[k]) : b->total[k];
+    const bool one = b->blocks[k] == 0 || b->blocks[k] >= b->total[k];
+    const unsigned grid = one ? b->total[k] : b->blocks[k];
+    const Fn fn = one ? fnOne : fnLoop;
     if (timing) (void)hipEventRecord(b->ev[2 * k], b->stream);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, b->stream, bl);
     if (timing) (void)hipEventRecord(b->ev[2 * k + 1], b->stream);
   };
-  launch(BK_DT, k_dt_reduce_batch<R>, cur);
+  launch(BK_DT, k_dt_reduce_batch<R, false>, k_dt_reduce_batch<R, true>, cur);
   {
     BatchLaunchT<R> bl{static_cast<const KPT<R>*>(b->dTable), nullptr, nullptr, n, cur, 0u, nullptr};
     hipLaunchKernelGGL(k_dt_finalize_batch<R>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, bl);
   }
-  launch(BK_NODE, k_lagrange_node_batch<R>, cur);
-  static void (*const cellK[3])(BatchLaunchT<R>) = {k_lagrange_cell_batch<R, 1>, k_lagrange_cell_batch<R, 2>,
-                                                     k_lagrange_cell_batch<R, 3>};
-  launch(BK_CELL, cellK[b->m - 1], cur);
+  launch(BK_NODE, k_lagrange_node_batch<R, false>, k_lagrange_node_batch<R, true>, cur);
+  static const Fn cellLoop[3] = {k_lagrange_cell_batch<R, 1, false>, k_lagrange_cell_batch<R, 2, false>,
+                                 k_lagrange_cell_batch<R, 3, false>};
+  static const Fn cellOne[3] = {k_lagrange_cell_batch<R, 1, true>, k_lagrange_cell_batch<R, 2, true>,
+                                k_lagrange_cell_batch<R, 3, true>};
+  launch(BK_CELL, cellLoop[b->m - 1], cellOne[b->m - 1], cur);
   for (int sweep = 1; sweep <= 2; ++sweep) {
     const bool xd = (sweep == 1) ? xty : !xty;
     const int fuse = (sweep == 2) ? 1 : 0;
     const int v = cur + 2 * (sweep - 1) + 4 * (xd ? 1 : 0);
-    launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R>(xd, b->m, b->pp), v + 8 * fuse);
-    if (xd) launch(BK_DUX, k_dual_wave_batch<R, true>, v);
-    else if (b->dualWaveY) launch(BK_DUY, k_dual_wave_batch<R, false>, v);
-    else launch(BK_DUY, k_dual_batch<R, false>, v);
+    launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R, false>(xd, b->m, b->pp), sweepBatchKernel<R, true>(xd, b->m, b->pp),
+           v + 8 * fuse);
+    if (xd) launch(BK_DUX, k_dual_wave_batch<R, true, false>, k_dual_wave_batch<R, true, true>, v);
+    else if (b->dualWaveY) launch(BK_DUY, k_dual_wave_batch<R, false, false>, k_dual_wave_batch<R, false, true>, v);
+    else launch(BK_DUY, k_dual_batch<R, false, false>, k_dual_batch<R, false, true>, v);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MHY_OK : batchFail(b, MHY_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));

@@ -19,17 +19,17 @@ template <class R>
 __global__ void k_lagrange_node(KPT<R> kp);
 template <class R, int M>
 __global__ void k_lagrange_cell(KPT<R> kp);
-template <class R>
+template <class R, bool ONE>
 __global__ void k_dt_reduce_batch(BatchLaunchT<R> bl);
-template <class R>
+template <class R, bool ONE>
 __global__ void k_lagrange_node_batch(BatchLaunchT<R> bl);
-template <class R, int M>
+template <class R, int M, bool ONE>
 __global__ void k_lagrange_cell_batch(BatchLaunchT<R> bl);
-template <class R, int M, bool PP, bool XD>
+template <class R, int M, bool PP, bool XD, bool ONE>
 __global__ void k_sweep_wave_batch(BatchLaunchT<R> bl);
-template <class R, bool XD>
+template <class R, bool XD, bool ONE>
 __global__ void k_dual_wave_batch(BatchLaunchT<R> bl);
-template <class R, bool XD>
+template <class R, bool XD, bool ONE>
 __global__ void k_dual_batch(BatchLaunchT<R> bl);
 // per-cell stencil sweeps: FP64 form only
 template <int M, bool PP, bool XD>

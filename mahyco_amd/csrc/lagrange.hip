@@ -411,9 +411,9 @@ __global__ __launch_bounds__(256) void k_lagrange_cell(KPT<R> kp) {
 // problems are numbered consecutively (BatchLaunch::start) and a fixed
 // number of workgroups walks them.
 
-template <class R>
+template <class R, bool ONE>
 __global__ __launch_bounds__(256) void k_dt_reduce_batch(BatchLaunchT<R> bl) {
-  batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned nb) {
+  batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned nb) {
     dtReduceBody(kp, b, nb);
     __syncthreads();  // the shared scratch is reused by the next virtual block
   });
@@ -426,15 +426,21 @@ __global__ __launch_bounds__(256) void k_dt_finalize_batch(BatchLaunchT<R> bl) {
   const KPT<R>& kp = bl.table[bl.variant * bl.nprob + p];
   dtFinalize(kp.f.sc, kp.cur, kp.nxt);
 }
-template <class R>
+template <class R, bool ONE>
 __global__ __launch_bounds__(256) void k_lagrange_node_batch(BatchLaunchT<R> bl) {
-  batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeNodeBody(kp, b); });
+  batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeNodeBody(kp, b); });
 }
-template <class R, int M>
+template <class R, int M, bool ONE>
 __global__ __launch_bounds__(256) void k_lagrange_cell_batch(BatchLaunchT<R> bl) {
-  batchForEach(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeCellBody<R, M>(kp, b); });
+  batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned b, unsigned, unsigned) { lagrangeCellBody<R, M>(kp, b); });
 }
 
+#define MHY_LAGRANGE_INST_ONE(R, ONE)                                        \
+  template __global__ void k_dt_reduce_batch<R, ONE>(BatchLaunchT<R>);       \
+  template __global__ void k_lagrange_node_batch<R, ONE>(BatchLaunchT<R>);   \
+  template __global__ void k_lagrange_cell_batch<R, 1, ONE>(BatchLaunchT<R>); \
+  template __global__ void k_lagrange_cell_batch<R, 2, ONE>(BatchLaunchT<R>); \
+  template __global__ void k_lagrange_cell_batch<R, 3, ONE>(BatchLaunchT<R>);
 #define MHY_LAGRANGE_INST(R)                                                 \
   template __global__ void k_dt_reduce<R>(KPT<R>);                           \
   template __global__ void k_dt_finalize<R>(KPT<R>);                         \
@@ -444,13 +450,11 @@ __global__ __launch_bounds__(256) void k_lagrange_cell_batch(BatchLaunchT<R> bl)
   template __global__ void k_lagrange_cell<R, 1>(KPT<R>);                    \
   template __global__ void k_lagrange_cell<R, 2>(KPT<R>);                    \
   template __global__ void k_lagrange_cell<R, 3>(KPT<R>);                    \
-  template __global__ void k_dt_reduce_batch<R>(BatchLaunchT<R>);            \
-  template __global__ void k_lagrange_node_batch<R>(BatchLaunchT<R>);        \
-  template __global__ void k_lagrange_cell_batch<R, 1>(BatchLaunchT<R>);     \
-  template __global__ void k_lagrange_cell_batch<R, 2>(BatchLaunchT<R>);     \
-  template __global__ void k_lagrange_cell_batch<R, 3>(BatchLaunchT<R>);
+  MHY_LAGRANGE_INST_ONE(R, false)                                            \
+  MHY_LAGRANGE_INST_ONE(R, true)
 MHY_LAGRANGE_INST(double)
 MHY_LAGRANGE_INST(float)
 #undef MHY_LAGRANGE_INST
+#undef MHY_LAGRANGE_INST_ONE
 
 }  // namespace mhy

@@ -186,9 +186,9 @@ template <class R, bool XD>
 __global__ __launch_bounds__(256) void k_dual(KPT<R> kp) {
   dualBody<R, XD>(kp, blockIdx.x);
 }
-template <class R, bool XD>
+template <class R, bool XD, bool ONE>
 __global__ __launch_bounds__(256) void k_dual_batch(BatchLaunchT<R> bl) {
-  batchForEach(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) { dualBody<R, XD>(kp, local); });
+  batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) { dualBody<R, XD>(kp, local); });
 }
 
 // The same nodal remap with one lane per node and the neighbour values taken
@@ -401,9 +401,9 @@ template <class R, bool XD>
 __global__ __launch_bounds__(256) void k_dual_wave(KPT<R> kp) {
   dualWaveBody<R, XD>(kp, blockIdx.x);
 }
-template <class R, bool XD>
+template <class R, bool XD, bool ONE>
 __global__ __launch_bounds__(256) void k_dual_wave_batch(BatchLaunchT<R> bl) {
-  batchForEach(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) { dualWaveBody<R, XD>(kp, local); });
+  batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) { dualWaveBody<R, XD>(kp, local); });
 }
 
 // launch grid of the wave dual kernels (256-thread blocks of 4 waves)
@@ -492,12 +492,16 @@ __global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
 #define MHY_REMAP_INST(R)                                              \
   template __global__ void k_dual<R, true>(KPT<R>);                    \
   template __global__ void k_dual<R, false>(KPT<R>);                   \
-  template __global__ void k_dual_batch<R, true>(BatchLaunchT<R>);     \
-  template __global__ void k_dual_batch<R, false>(BatchLaunchT<R>);    \
+  template __global__ void k_dual_batch<R, true, false>(BatchLaunchT<R>);  \
+  template __global__ void k_dual_batch<R, false, false>(BatchLaunchT<R>); \
+  template __global__ void k_dual_batch<R, true, true>(BatchLaunchT<R>);   \
+  template __global__ void k_dual_batch<R, false, true>(BatchLaunchT<R>);  \
   template __global__ void k_dual_wave<R, true>(KPT<R>);               \
   template __global__ void k_dual_wave<R, false>(KPT<R>);              \
-  template __global__ void k_dual_wave_batch<R, true>(BatchLaunchT<R>); \
-  template __global__ void k_dual_wave_batch<R, false>(BatchLaunchT<R>); \
+  template __global__ void k_dual_wave_batch<R, true, false>(BatchLaunchT<R>);  \
+  template __global__ void k_dual_wave_batch<R, false, false>(BatchLaunchT<R>); \
+  template __global__ void k_dual_wave_batch<R, true, true>(BatchLaunchT<R>);   \
+  template __global__ void k_dual_wave_batch<R, false, true>(BatchLaunchT<R>);  \
   template __global__ void k_remap_final<R, 1>(KPT<R>);                \
   template __global__ void k_remap_final<R, 2>(KPT<R>);                \
   template __global__ void k_remap_final<R, 3>(KPT<R>);                \

@@ -339,9 +339,9 @@ __global__ __launch_bounds__(256) void k_sweep_wave(KPT<R> kp) {
 }
 
 // all problems of an ensemble in one launch (BatchLaunchT, common.hpp)
-template <class R, int M, bool PP, bool XD>
+template <class R, int M, bool PP, bool XD, bool ONE>
 __global__ __launch_bounds__(256) void k_sweep_wave_batch(BatchLaunchT<R> bl) {
-  batchForEach(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) {
+  batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) {
     sweepWaveBody<R, M, PP, XD>(kp, local);
   });
 }
@@ -349,8 +349,10 @@ __global__ __launch_bounds__(256) void k_sweep_wave_batch(BatchLaunchT<R> bl) {
 #define MHY_SWEEP_WAVE_INST(R, M, PP)                                          \
   template __global__ void k_sweep_wave<R, M, PP, true>(KPT<R>);               \
   template __global__ void k_sweep_wave<R, M, PP, false>(KPT<R>);              \
-  template __global__ void k_sweep_wave_batch<R, M, PP, true>(BatchLaunchT<R>); \
-  template __global__ void k_sweep_wave_batch<R, M, PP, false>(BatchLaunchT<R>);
+  template __global__ void k_sweep_wave_batch<R, M, PP, true, false>(BatchLaunchT<R>);  \
+  template __global__ void k_sweep_wave_batch<R, M, PP, false, false>(BatchLaunchT<R>); \
+  template __global__ void k_sweep_wave_batch<R, M, PP, true, true>(BatchLaunchT<R>);   \
+  template __global__ void k_sweep_wave_batch<R, M, PP, false, true>(BatchLaunchT<R>);
 #define MHY_SWEEP_WAVE_INST_R(R)    \
   MHY_SWEEP_WAVE_INST(R, 1, false) \
   MHY_SWEEP_WAVE_INST(R, 1, true)  \
