@@ -265,25 +265,26 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
   kp.xdir = xd ? 1 : 0;
   if (stage == MHY_STAGE_CELLS) {
-    const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny) : gridFor(x->ncell);
+    const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny, x->mixed) : gridFor(x->ncell);
+    const dim3 block(x->sweepWave ? sweepWaveBlock(xd, x->mixed) : kBlock);
     if (x->mixed) {
       const KernelFnF k = sweepWaveKernel<float>(xd, x->o.nbmat, x->o.pp == 1);
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
-            [&] { hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, x->stream, retype<float>(kp)); });
+            [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, retype<float>(kp)); });
     } else {
       const KernelFn k = sweepKernel(x->sweepWave, xd, x->o.nbmat, x->o.pp == 1);
-      timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, x->stream, kp); });
+      timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, kp); });
     }
   } else if (x->sweepWave && xd) {
-    const dim3 grid = dualWaveGrid(true, x->o.nx, x->o.ny);
+    const dim3 grid = dualWaveGrid(true, x->o.nx, x->o.ny, x->mixed);
     timed(x, K_DUAL_X, [&] { launchR(x, k_dual_wave<double, true>, k_dual_wave<float, true>, grid, dim3(kBlock), kp); });
   } else if (xd) {
     timed(x, K_DUAL_X,
           [&] { launchR(x, k_dual<double, true>, k_dual<float, true>, gridFor(x->nnode), dim3(kBlock), kp); });
   } else if (x->sweepWave && x->dualWaveY) {
-    const dim3 grid = dualWaveGrid(false, x->o.nx, x->o.ny);
+    const dim3 grid = dualWaveGrid(false, x->o.nx, x->o.ny, x->mixed);
     timed(x, K_DUAL_Y,
-          [&] { launchR(x, k_dual_wave<double, false>, k_dual_wave<float, false>, grid, dim3(kBlock), kp); });
+          [&] { launchR(x, k_dual_wave<double, false>, k_dual_wave<float, false>, grid, dim3(dualWaveBlock(false, x->mixed)), kp); });
   } else {
     timed(x, K_DUAL_Y,
           [&] { launchR(x, k_dual<double, false>, k_dual<float, false>, gridFor(x->nnode), dim3(kBlock), kp); });
@@ -975,14 +976,14 @@ int batchEnqueueStepR(mhy_batch* b, int cur, bool xty, bool timing) {
   // fnLoop walks contiguous virtual-block ranges (blocks[k] workgroups);
   // fnOne runs one workgroup per virtual block (blocks[k] == 0)
   using Fn = void (*)(BatchLaunchT<R>);
-  auto launch = [&](int k, Fn fnLoop, Fn fnOne, int variant) {
+  auto launch = [&](int k, Fn fnLoop, Fn fnOne, int variant, unsigned block = kBlock) {
     BatchLaunchT<R> bl{static_cast<const KPT<R>*>(b->dTable), b->dStart + (size_t)k * (n + 1),
                        b->dGx + (size_t)k * n, n, variant, b->total[k], b->dOwner + b->ownerOff[k]};
     const bool one = b->blocks[k] == 0 || b->blocks[k] >= b->total[k];
     const unsigned grid = one ? b->total[k] : b->blocks[k];
     const Fn fn = one ? fnOne : fnLoop;
     if (timing) (void)hipEventRecord(b->ev[2 * k], b->stream);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, b->stream, bl);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, b->stream, bl);
     if (timing) (void)hipEventRecord(b->ev[2 * k + 1], b->stream);
   };
   launch(BK_DT, k_dt_reduce_batch<R, false>, k_dt_reduce_batch<R, true>, cur);
@@ -1001,9 +1002,10 @@ int batchEnqueueStepR(mhy_batch* b, int cur, bool xty, bool timing) {
     const int fuse = (sweep == 2) ? 1 : 0;
     const int v = cur + 2 * (sweep - 1) + 4 * (xd ? 1 : 0);
     launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R, false>(xd, b->m, b->pp), sweepBatchKernel<R, true>(xd, b->m, b->pp),
-           v + 8 * fuse);
+           v + 8 * fuse, sweepWaveBlock(xd, b->mixed));
     if (xd) launch(BK_DUX, k_dual_wave_batch<R, true, false>, k_dual_wave_batch<R, true, true>, v);
-    else if (b->dualWaveY) launch(BK_DUY, k_dual_wave_batch<R, false, false>, k_dual_wave_batch<R, false, true>, v);
+    else if (b->dualWaveY)
+      launch(BK_DUY, k_dual_wave_batch<R, false, false>, k_dual_wave_batch<R, false, true>, v, dualWaveBlock(false, b->mixed));
     else launch(BK_DUY, k_dual_batch<R, false, false>, k_dual_batch<R, false, true>, v);
   }
   hipError_t e = hipGetLastError();
@@ -1108,10 +1110,10 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
           break;
         case BK_NODE: g = gridFor(x->nnode); break;
         case BK_CELL: g = gridFor(x->ncell); break;
-        case BK_SWX: g = sweepWaveGrid(true, nx, ny); break;
-        case BK_SWY: g = sweepWaveGrid(false, nx, ny); break;
-        case BK_DUX: g = dualWaveGrid(true, nx, ny); break;
-        case BK_DUY: g = b->dualWaveY ? dualWaveGrid(false, nx, ny) : gridFor(x->nnode); break;
+        case BK_SWX: g = sweepWaveGrid(true, nx, ny, b->mixed); break;
+        case BK_SWY: g = sweepWaveGrid(false, nx, ny, b->mixed); break;
+        case BK_DUX: g = dualWaveGrid(true, nx, ny, b->mixed); break;
+        case BK_DUY: g = b->dualWaveY ? dualWaveGrid(false, nx, ny, b->mixed) : gridFor(x->nnode); break;
         default: g = gridFor(x->nnode); break;  // per-node dual along y
       }
       start[(size_t)k * (n + 1) + i] = acc;

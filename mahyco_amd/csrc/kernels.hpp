@@ -36,12 +36,14 @@ template <int M, bool PP, bool XD>
 __global__ void k_sweep_cell(KP kp);
 template <class R, int M, bool PP, bool XD>
 __global__ void k_sweep_wave(KPT<R> kp);
-dim3 sweepWaveGrid(bool xd, long nx, long ny);
+dim3 sweepWaveGrid(bool xd, long nx, long ny, bool mixed);
+unsigned sweepWaveBlock(bool xd, bool mixed);
 template <class R, bool XD>
 __global__ void k_dual(KPT<R> kp);
 template <class R, bool XD>
 __global__ void k_dual_wave(KPT<R> kp);
-dim3 dualWaveGrid(bool xd, long nx, long ny);
+dim3 dualWaveGrid(bool xd, long nx, long ny, bool mixed);
+unsigned dualWaveBlock(bool xd, bool mixed);
 template <class R, int M>
 __global__ void k_remap_final(KPT<R> kp);
 template <class R>

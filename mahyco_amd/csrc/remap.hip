@@ -202,6 +202,9 @@ __global__ __launch_bounds__(256) void k_dual_batch(BatchLaunchT<R> bl) {
 namespace {
 constexpr int kDW = 64;
 constexpr int kDYCols = 4;
+// waves per block along y: 16 (FP64) or 32 (FP32) node columns
+template <class R>
+constexpr int kDYWaves = kMixed<R> ? 8 : 4;
 constexpr int kDYRows = kDW / kDYCols;
 constexpr int kDXOut = kDW - 6;
 constexpr int kDYOut = kDYRows - 6;
@@ -231,11 +234,12 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
     t = g / nseg;
     na = nx, nt = ny;  // node along index in [0, na], across in [0, nt]
   } else {
-    const long gx = (nx + 1 + 4 * kDYCols - 1) / (4 * kDYCols);
+    constexpr int W = kDYWaves<R>;
+    const long gx = (nx + 1 + W * kDYCols - 1) / (W * kDYCols);
     const long bx = b % gx, by = b / gx;
     slot = lane / kDYCols, last = kDYRows - 1;
     a = by * kDYOut - 3 + slot;
-    t = bx * (4 * kDYCols) + wave * kDYCols + (lane % kDYCols);
+    t = bx * (W * kDYCols) + wave * kDYCols + (lane % kDYCols);
     na = ny, nt = nx;
   }
   const bool in = (a >= 0 && a <= na && t >= 0 && t <= nt);
@@ -398,21 +402,28 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
 }
 
 template <class R, bool XD>
-__global__ __launch_bounds__(256) void k_dual_wave(KPT<R> kp) {
+__global__ __launch_bounds__(XD ? 256 : 64 * kDYWaves<R>) void k_dual_wave(KPT<R> kp) {
   dualWaveBody<R, XD>(kp, blockIdx.x);
 }
 template <class R, bool XD, bool ONE>
-__global__ __launch_bounds__(256) void k_dual_wave_batch(BatchLaunchT<R> bl) {
+__global__ __launch_bounds__(XD ? 256 : 64 * kDYWaves<R>) void k_dual_wave_batch(BatchLaunchT<R> bl) {
   batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) { dualWaveBody<R, XD>(kp, local); });
 }
 
-// launch grid of the wave dual kernels (256-thread blocks of 4 waves)
-dim3 dualWaveGrid(bool xd, long nx, long ny) {
+// threads per block of the wave dual kernels: 4 waves along x; along y 4
+// (FP64) or 8 (mixed form)
+unsigned dualWaveBlock(bool xd, bool mixed) {
+  return xd ? 256u : 64u * (mixed ? kDYWaves<float> : kDYWaves<double>);
+}
+
+// launch grid of the wave dual kernels (blocks of dualWaveBlock threads)
+dim3 dualWaveGrid(bool xd, long nx, long ny, bool mixed) {
   if (xd) {
     const long strips = (nx + 1 + kDXOut - 1) / kDXOut * (ny + 1);
     return dim3((unsigned)((strips + 3) / 4));
   }
-  const long gx = (nx + 1 + 4 * kDYCols - 1) / (4 * kDYCols), gy = (ny + 1 + kDYOut - 1) / kDYOut;
+  const int W = mixed ? kDYWaves<float> : kDYWaves<double>;
+  const long gx = (nx + 1 + W * kDYCols - 1) / (W * kDYCols), gy = (ny + 1 + kDYOut - 1) / kDYOut;
   return dim3((unsigned)(gx * gy));
 }
 
