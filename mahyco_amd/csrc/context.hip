@@ -265,8 +265,8 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
   kp.xdir = xd ? 1 : 0;
   if (stage == MHY_STAGE_CELLS) {
-    const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny, x->mixed) : gridFor(x->ncell);
-    const dim3 block(x->sweepWave ? sweepWaveBlock(xd, x->mixed) : kBlock);
+    const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny, x->mixed, x->o.pp == 1) : gridFor(x->ncell);
+    const dim3 block(x->sweepWave ? sweepWaveBlock(xd, x->mixed, x->o.pp == 1) : kBlock);
     if (x->mixed) {
       const KernelFnF k = sweepWaveKernel<float>(xd, x->o.nbmat, x->o.pp == 1);
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
@@ -1002,7 +1002,7 @@ int batchEnqueueStepR(mhy_batch* b, int cur, bool xty, bool timing) {
     const int fuse = (sweep == 2) ? 1 : 0;
     const int v = cur + 2 * (sweep - 1) + 4 * (xd ? 1 : 0);
     launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R, false>(xd, b->m, b->pp), sweepBatchKernel<R, true>(xd, b->m, b->pp),
-           v + 8 * fuse, sweepWaveBlock(xd, b->mixed));
+           v + 8 * fuse, sweepWaveBlock(xd, b->mixed, b->pp));
     if (xd) launch(BK_DUX, k_dual_wave_batch<R, true, false>, k_dual_wave_batch<R, true, true>, v);
     else if (b->dualWaveY)
       launch(BK_DUY, k_dual_wave_batch<R, false, false>, k_dual_wave_batch<R, false, true>, v, dualWaveBlock(false, b->mixed));
@@ -1110,8 +1110,8 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
           break;
         case BK_NODE: g = gridFor(x->nnode); break;
         case BK_CELL: g = gridFor(x->ncell); break;
-        case BK_SWX: g = sweepWaveGrid(true, nx, ny, b->mixed); break;
-        case BK_SWY: g = sweepWaveGrid(false, nx, ny, b->mixed); break;
+        case BK_SWX: g = sweepWaveGrid(true, nx, ny, b->mixed, b->pp); break;
+        case BK_SWY: g = sweepWaveGrid(false, nx, ny, b->mixed, b->pp); break;
         case BK_DUX: g = dualWaveGrid(true, nx, ny, b->mixed); break;
         case BK_DUY: g = b->dualWaveY ? dualWaveGrid(false, nx, ny, b->mixed) : gridFor(x->nnode); break;
         default: g = gridFor(x->nnode); break;  // per-node dual along y

@@ -36,8 +36,8 @@ template <int M, bool PP, bool XD>
 __global__ void k_sweep_cell(KP kp);
 template <class R, int M, bool PP, bool XD>
 __global__ void k_sweep_wave(KPT<R> kp);
-dim3 sweepWaveGrid(bool xd, long nx, long ny, bool mixed);
-unsigned sweepWaveBlock(bool xd, bool mixed);
+dim3 sweepWaveGrid(bool xd, long nx, long ny, bool mixed, bool pp);
+unsigned sweepWaveBlock(bool xd, bool mixed, bool pp);
 template <class R, bool XD>
 __global__ void k_dual(KPT<R> kp);
 template <class R, bool XD>

@@ -37,10 +37,11 @@ constexpr int kWave = 64;
 constexpr int kHalo = 2;                // halo cells at each end of a strip
 constexpr int kXOut = kWave - 2 * kHalo;   // cells updated per wave along x
 constexpr int kYCols = 4;               // columns per wave along y
-// waves per block along y: 16 (FP64) or 32 (FP32) columns, so that a block
-// writes whole 128-byte lines of each row
-template <class R>
-constexpr int kYWaves = kMixed<R> ? 8 : 4;
+// waves per block along y: 16 columns, or 32 in the mixed form's
+// limited-slope sweeps (FP32 rows, few registers), so that a block writes
+// whole 128-byte lines of each row
+template <class R, bool PP>
+constexpr int kYWaves = (kMixed<R> && !PP) ? 8 : 4;
 constexpr int kYRows = kWave / kYCols;
 constexpr int kYOut = kYRows - 2 * kHalo;  // rows updated per wave along y
 
@@ -106,7 +107,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     t = g / nseg;  // rows beyond ny have no cells
     na = nx, nt = ny;
   } else {
-    constexpr int W = kYWaves<R>;
+    constexpr int W = kYWaves<R, PP>;
     const long gx = (nx + W * kYCols - 1) / (W * kYCols);
     const long bx = b % gx, by = b / gx;
     slot = lane / kYCols;
@@ -339,13 +340,13 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 }
 
 template <class R, int M, bool PP, bool XD>
-__global__ __launch_bounds__(XD ? 256 : 64 * kYWaves<R>) void k_sweep_wave(KPT<R> kp) {
+__global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>)) void k_sweep_wave(KPT<R> kp) {
   sweepWaveBody<R, M, PP, XD>(kp, XD ? blockIdx.x : xcdBlock());
 }
 
 // all problems of an ensemble in one launch (BatchLaunchT, common.hpp)
 template <class R, int M, bool PP, bool XD, bool ONE>
-__global__ __launch_bounds__(XD ? 256 : 64 * kYWaves<R>) void k_sweep_wave_batch(BatchLaunchT<R> bl) {
+__global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>)) void k_sweep_wave_batch(BatchLaunchT<R> bl) {
   batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) {
     sweepWaveBody<R, M, PP, XD>(kp, local);
   });
@@ -370,19 +371,19 @@ MHY_SWEEP_WAVE_INST_R(float)
 #undef MHY_SWEEP_WAVE_INST_R
 #undef MHY_SWEEP_WAVE_INST
 
-// threads per block of the wave sweeps: 4 waves along x; along y 4 (FP64)
-// or 8 (mixed form)
-unsigned sweepWaveBlock(bool xd, bool mixed) {
-  return xd ? 256u : 64u * (mixed ? kYWaves<float> : kYWaves<double>);
+// threads per block of the wave sweeps: 4 waves along x; along y 4, or 8
+// in the mixed form's limited-slope sweeps
+unsigned sweepWaveBlock(bool xd, bool mixed, bool pp) {
+  return xd ? 256u : 64u * ((mixed && !pp) ? kYWaves<float, false> : kYWaves<double, false>);
 }
 
 // launch grid of the wave sweeps (blocks of sweepWaveBlock threads)
-dim3 sweepWaveGrid(bool xd, long nx, long ny, bool mixed) {
+dim3 sweepWaveGrid(bool xd, long nx, long ny, bool mixed, bool pp) {
   if (xd) {
     const long strips = (nx + kXOut - 1) / kXOut * ny;
     return dim3((unsigned)((strips + 3) / 4));
   }
-  const int W = mixed ? kYWaves<float> : kYWaves<double>;
+  const int W = (mixed && !pp) ? kYWaves<float, false> : kYWaves<double, false>;
   const long gx = (nx + W * kYCols - 1) / (W * kYCols), gy = (ny + kYOut - 1) / kYOut;
   return dim3((unsigned)(gx * gy));
 }
