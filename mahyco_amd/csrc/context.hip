@@ -54,6 +54,12 @@ struct mhy_ctx {
   // nodal remap along y: the wave kernel (default) or the per-node stencil
   // kernel (environment MAHYCO_DUAL_Y=node)
   bool dualWaveY = true;
+  // cell sweeps along y: the column-walking kernel or the wave-strip
+  // kernel.  The walker keeps more state in registers (fewer waves per SIMD)
+  // but has less halo work; it is chosen for the mixed form with one
+  // material and limited slopes on meshes of at least 4096 walking waves,
+  // where it measured faster.  MAHYCO_SWEEP_Y=walk / wave forces either.
+  bool walkY = false;
   int status = MHY_OK;
   long ncell = 0, nnode = 0, nface = 0;
   std::string err;
@@ -267,7 +273,18 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   if (stage == MHY_STAGE_CELLS) {
     const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny, x->mixed, x->o.pp == 1) : gridFor(x->ncell);
     const dim3 block(x->sweepWave ? sweepWaveBlock(xd, x->mixed, x->o.pp == 1) : kBlock);
-    if (x->mixed) {
+    if (!xd && x->sweepWave && x->walkY) {
+      static const KernelFn kd[3][2] = {{k_sweep_walk<double, 1, false>, k_sweep_walk<double, 1, true>},
+                                        {k_sweep_walk<double, 2, false>, k_sweep_walk<double, 2, true>},
+                                        {k_sweep_walk<double, 3, false>, k_sweep_walk<double, 3, true>}};
+      static const KernelFnF kf[3][2] = {{k_sweep_walk<float, 1, false>, k_sweep_walk<float, 1, true>},
+                                         {k_sweep_walk<float, 2, false>, k_sweep_walk<float, 2, true>},
+                                         {k_sweep_walk<float, 3, false>, k_sweep_walk<float, 3, true>}};
+      const int mi = x->o.nbmat - 1, pi = x->o.pp == 1 ? 1 : 0;
+      timed(x, K_SWEEP_Y, [&] {
+        launchR(x, kd[mi][pi], kf[mi][pi], sweepWalkGrid(x->o.nx, x->o.ny), dim3(kBlock), kp);
+      });
+    } else if (x->mixed) {
       const KernelFnF k = sweepWaveKernel<float>(xd, x->o.nbmat, x->o.pp == 1);
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
             [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, retype<float>(kp)); });
@@ -436,6 +453,11 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     return bail(fail(x, MHY_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed FP32 / FP64)"));
   x->mixed = (c.precision == 1);
   if (x->mixed) x->sweepWave = true;  // the per-cell stencil sweeps are FP64 only
+  x->walkY = x->mixed && c.nbmat == 1 && c.pp != 1 && (long)sweepWalkGrid(c.nx, c.ny).x * 4 >= 4096;
+  if (const char* wy = std::getenv("MAHYCO_SWEEP_Y")) {
+    if (std::string(wy) == "walk") x->walkY = true;
+    if (std::string(wy) == "wave") x->walkY = false;
+  }
   if (c.sans_lagrange && !c.avec_projection)
     return bail(fail(x, MHY_ERR_ARG, "sansLagrange needs the projection"));
   std::string err;
@@ -933,6 +955,14 @@ void (*sweepBatchKernel(bool xd, int m, bool pp))(BatchLaunchT<R>) {
        {k_sweep_wave_batch<R, 3, false, true, ONE>, k_sweep_wave_batch<R, 3, true, true, ONE>}}};
   return t[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
+template <class R, bool ONE>
+void (*walkBatchKernel(int m, bool pp))(BatchLaunchT<R>) {
+  using Fn = void (*)(BatchLaunchT<R>);
+  static const Fn t[3][2] = {{k_sweep_walk_batch<R, 1, false, ONE>, k_sweep_walk_batch<R, 1, true, ONE>},
+                             {k_sweep_walk_batch<R, 2, false, ONE>, k_sweep_walk_batch<R, 2, true, ONE>},
+                             {k_sweep_walk_batch<R, 3, false, ONE>, k_sweep_walk_batch<R, 3, true, ONE>}};
+  return t[m - 1][pp ? 1 : 0];
+}
 }  // namespace
 
 struct mhy_batch {
@@ -942,6 +972,7 @@ struct mhy_batch {
   bool pp = false;
   bool mixed = false;  // every problem in the mixed FP32 / FP64 form
   bool dualWaveY = true;  // nodal remap along y with the wave kernel
+  bool walkY = false;     // cell sweeps along y with the column-walking kernel
   void* dTable = nullptr;  // KPT<double> or KPT<float> entries
   unsigned* dStart = nullptr;  // BK_COUNT x (n+1)
   unsigned* dGx = nullptr;     // BK_COUNT x n
@@ -1001,8 +1032,11 @@ int batchEnqueueStepR(mhy_batch* b, int cur, bool xty, bool timing) {
     const bool xd = (sweep == 1) ? xty : !xty;
     const int fuse = (sweep == 2) ? 1 : 0;
     const int v = cur + 2 * (sweep - 1) + 4 * (xd ? 1 : 0);
-    launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R, false>(xd, b->m, b->pp), sweepBatchKernel<R, true>(xd, b->m, b->pp),
-           v + 8 * fuse, sweepWaveBlock(xd, b->mixed, b->pp));
+    if (!xd && b->walkY)
+      launch(BK_SWY, walkBatchKernel<R, false>(b->m, b->pp), walkBatchKernel<R, true>(b->m, b->pp), v + 8 * fuse);
+    else
+      launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R, false>(xd, b->m, b->pp),
+             sweepBatchKernel<R, true>(xd, b->m, b->pp), v + 8 * fuse, sweepWaveBlock(xd, b->mixed, b->pp));
     if (xd) launch(BK_DUX, k_dual_wave_batch<R, true, false>, k_dual_wave_batch<R, true, true>, v);
     else if (b->dualWaveY)
       launch(BK_DUY, k_dual_wave_batch<R, false, false>, k_dual_wave_batch<R, false, true>, v, dualWaveBlock(false, b->mixed));
@@ -1066,6 +1100,7 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
   b->pp = cfgs[0].pp == 1;
   b->mixed = cfgs[0].precision == 1;
   if (const char* dy = std::getenv("MAHYCO_DUAL_Y")) b->dualWaveY = (std::string(dy) != "node");
+  if (const char* wy = std::getenv("MAHYCO_SWEEP_Y")) b->walkY = (std::string(wy) == "walk");
   if (const char* g = std::getenv("MAHYCO_BATCH_GRAPH")) b->useGraph = (std::string(g) != "0");
   if (const char* t = std::getenv("MAHYCO_BATCH_TUNE")) b->tune = (std::string(t) != "0");
   b->ctx.resize(n, nullptr);
@@ -1111,7 +1146,7 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
         case BK_NODE: g = gridFor(x->nnode); break;
         case BK_CELL: g = gridFor(x->ncell); break;
         case BK_SWX: g = sweepWaveGrid(true, nx, ny, b->mixed, b->pp); break;
-        case BK_SWY: g = sweepWaveGrid(false, nx, ny, b->mixed, b->pp); break;
+        case BK_SWY: g = b->walkY ? sweepWalkGrid(nx, ny) : sweepWaveGrid(false, nx, ny, b->mixed, b->pp); break;
         case BK_DUX: g = dualWaveGrid(true, nx, ny, b->mixed); break;
         case BK_DUY: g = b->dualWaveY ? dualWaveGrid(false, nx, ny, b->mixed) : gridFor(x->nnode); break;
         default: g = gridFor(x->nnode); break;  // per-node dual along y

@@ -38,6 +38,12 @@ template <class R, int M, bool PP, bool XD>
 __global__ void k_sweep_wave(KPT<R> kp);
 dim3 sweepWaveGrid(bool xd, long nx, long ny, bool mixed, bool pp);
 unsigned sweepWaveBlock(bool xd, bool mixed, bool pp);
+// y sweeps with one lane per column walking down a band of rows
+template <class R, int M, bool PP>
+__global__ void k_sweep_walk(KPT<R> kp);
+template <class R, int M, bool PP, bool ONE>
+__global__ void k_sweep_walk_batch(BatchLaunchT<R> bl);
+dim3 sweepWalkGrid(long nx, long ny);
 template <class R, bool XD>
 __global__ void k_dual(KPT<R> kp);
 template <class R, bool XD>
