@@ -11,6 +11,16 @@ namespace mhy {
 // first error wins (MHY_ERR_*), read by the host after the step
 __device__ __forceinline__ void raiseErr(Scalars* sc, int code) { atomicCAS(&sc->err, 0, code); }
 
+// What the final remap stores for one cell, already in the storage types.
+template <class R, int M>
+struct FinalOut {
+  R rho, e, q, p, c;
+  int mixte, pure;
+  double Mc;
+  double fr[M], Mk[M];
+  R mu[M], rk[M], ek[M], pk[M], ck[M];
+};
+
 // U: the 3m+1 live components of the remapped vector U2 of cell c.
 // The volume fractions near 0 and 1, their threshold tests, the densities,
 // specific energies and masses are FP64 in both forms (a nearly emptied
@@ -18,7 +28,7 @@ __device__ __forceinline__ void raiseErr(Scalars* sc, int code) { atomicCAS(&sc-
 // mixed form (R = float) takes the EOS sound speeds' square roots in FP32,
 // the precision they are stored in.
 template <class R, int M>
-__device__ __forceinline__ void finalCell(const KPT<R>& kp, long c, const double* U) {
+__device__ __forceinline__ void finalCompute(const KPT<R>& kp, long c, const double* U, FinalOut<R, M>& out) {
   const Opts& o = kp.o;
   const auto& f = kp.f;
   const int cur = kp.cur, nxt = kp.nxt;
@@ -129,24 +139,55 @@ __device__ __forceinline__ void finalCell(const KPT<R>& kp, long c, const double
 #pragma unroll
   for (int k = 0; k < M; ++k)
     if (o.eos[k] == MHY_EOS_VOID) e1 += mu[k] * ek[k];
-  f.rho[nxt][c] = rho1;
-  f.e[nxt][c] = e1;
-  f.q[nxt][c] = q1;
-  f.p[nxt][c] = p1;
-  f.c[nxt][c] = c1;
-  f.M[c] = Mc;
-  f.mixte[c] = mixte;
-  f.pure[c] = pure;
+  out.rho = (R)rho1;
+  out.e = (R)e1;
+  out.q = (R)q1;
+  out.p = (R)p1;
+  out.c = (R)c1;
+  out.Mc = Mc;
+  out.mixte = mixte;
+  out.pure = pure;
 #pragma unroll
   for (int k = 0; k < M; ++k) {
-    f.f_k[k][c] = fr[k];
-    f.mu_k[k][c] = mu[k];
-    f.M_k[k][c] = Mk[k];
-    f.rho_k[nxt][k][c] = rk[k];
-    f.e_k[nxt][k][c] = ek[k];
-    f.p_k[nxt][k][c] = pk[k];
-    f.c_k[nxt][k][c] = ck[k];
+    out.fr[k] = fr[k];
+    out.mu[k] = (R)mu[k];
+    out.Mk[k] = Mk[k];
+    out.rk[k] = (R)rk[k];
+    out.ek[k] = (R)ek[k];
+    out.pk[k] = (R)pk[k];
+    out.ck[k] = (R)ck[k];
   }
+}
+
+template <class R, int M>
+__device__ __forceinline__ void finalStore(const KPT<R>& kp, long c, const FinalOut<R, M>& out) {
+  const auto& f = kp.f;
+  const int nxt = kp.nxt;
+  f.rho[nxt][c] = out.rho;
+  f.e[nxt][c] = out.e;
+  f.q[nxt][c] = out.q;
+  f.p[nxt][c] = out.p;
+  f.c[nxt][c] = out.c;
+  f.M[c] = out.Mc;
+  f.mixte[c] = out.mixte;
+  f.pure[c] = out.pure;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    f.f_k[k][c] = out.fr[k];
+    f.mu_k[k][c] = out.mu[k];
+    f.M_k[k][c] = out.Mk[k];
+    f.rho_k[nxt][k][c] = out.rk[k];
+    f.e_k[nxt][k][c] = out.ek[k];
+    f.p_k[nxt][k][c] = out.pk[k];
+    f.c_k[nxt][k][c] = out.ck[k];
+  }
+}
+
+template <class R, int M>
+__device__ __forceinline__ void finalCell(const KPT<R>& kp, long c, const double* U) {
+  FinalOut<R, M> out;
+  finalCompute<R, M>(kp, c, U, out);
+  finalStore<R, M>(kp, c, out);
 }
 
 }  // namespace mhy

@@ -6,6 +6,8 @@
 // Shared by the wave-strip sweeps and the column-walking y sweeps.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "final_math.hpp"
 
@@ -21,17 +23,18 @@ __device__ __forceinline__ void twoSumAcc(double& s, double& comp, double v) {
 
 // Cell c (along index a, row j, column i) with face values phiM / phiP at
 // its minus / plus faces (valid when hasM / hasP), face normal velocities
-// vm / vp and Euler face lengths Lm / Lp.
+// vm / vp and Euler face lengths Lm / Lp: its new remap vector Unew and the
+// flux of its minus face for the dual remap (A1: mass, A2: volume; zero for
+// PB).  The flux of the last face of a line is stored here.
 template <class R, int M, bool PP, bool XD, class T>
-__device__ __forceinline__ void sweepUpdateCell(const KPT<R>& kp, long c, long i, long j, long a, bool hasM,
-                                                bool hasP, double vm, double vp, double Lm, double Lp,
-                                                const T* phiM, const T* phiP) {
+__device__ __forceinline__ void sweepUnew(const KPT<R>& kp, long c, long i, long j, long a, bool hasM, bool hasP,
+                                          double vm, double vp, double Lm, double Lp, const T* phiM, const T* phiP,
+                                          double* Unew, double* Gdual) {
   constexpr int NC = 3 * M + 1;
   const Opts& o = kp.o;
   const auto& f = kp.f;
   const long nx = o.nx, ny = o.ny;
   double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
-  double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
   const double dtn = f.sc->dt[kp.cur];
   // computeUremap (PhaseRemap1.cc:468-503): a cell receives -G through its
   // minus face and +G through its plus face
@@ -72,7 +75,6 @@ __device__ __forceinline__ void sweepUpdateCell(const KPT<R>& kp, long c, long i
       }
     }
   }
-  double Unew[NC];
   if (kMixed<R>) {
     // U - (-Gm + Gp + Fb), compensated: the FP64 result is the correctly
     // rounded sum up to a final rounding, whatever the flux magnitudes
@@ -88,26 +90,139 @@ __device__ __forceinline__ void sweepUpdateCell(const KPT<R>& kp, long c, long i
 #pragma unroll
     for (int q = 0; q < NC; ++q) Unew[q] = Uin[q][c] - red[q];
   }
+  const bool withG = (o.methode == 0 || o.methode == 1);
+  const int q0 = (o.methode == 0) ? M : 0;
+#pragma unroll
+  for (int k = 0; k < M; ++k) Gdual[k] = (withG && hasM) ? Gm[q0 + k] : 0.0;
+  if (withG && !hasP) {
+    const long fp = XD ? j * (nx + 1) + a + 1 : (a + 1) * nx + i;
+#pragma unroll
+    for (int k = 0; k < M; ++k) f.G[k][fp] = 0.0;
+  }
+}
+
+// The update of cell c and its stores: the final remap of the cell from
+// registers (second sweep, fused) or the new remap vector, and the flux of
+// its minus face for the dual remap.
+template <class R, int M, bool PP, bool XD, class T>
+__device__ __forceinline__ void sweepUpdateCell(const KPT<R>& kp, long c, long i, long j, long a, bool hasM,
+                                                bool hasP, double vm, double vp, double Lm, double Lp,
+                                                const T* phiM, const T* phiP) {
+  constexpr int NC = 3 * M + 1;
+  const auto& f = kp.f;
+  double Unew[NC], Gd[M];
+  sweepUnew<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, Gd);
   if (kp.fuseFinal) {
     finalCell<R, M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
   } else {
+    double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
 #pragma unroll
     for (int q = 0; q < NC; ++q) Uout[q][c] = Unew[q];
   }
-
-  // face fluxes for the dual remap (A1: mass, A2: volume): the minus face of
-  // each cell, plus the last face of the line
-  if (o.methode == 0 || o.methode == 1) {
-    const int q0 = (o.methode == 0) ? M : 0;
-    const long fm = XD ? j * (nx + 1) + a : a * nx + i;
+  if (kp.o.methode == 0 || kp.o.methode == 1) {
+    const long fm = XD ? j * (kp.o.nx + 1) + a : a * kp.o.nx + i;
 #pragma unroll
-    for (int k = 0; k < M; ++k) f.G[k][fm] = hasM ? Gm[q0 + k] : 0.0;
-    if (!hasP) {
-      const long fp = XD ? j * (nx + 1) + a + 1 : (a + 1) * nx + i;
-#pragma unroll
-      for (int k = 0; k < M; ++k) f.G[k][fp] = 0.0;
-    }
+    for (int k = 0; k < M; ++k) f.G[k][fm] = Gd[k];
   }
 }
+
+// What a sweep stores for one cell: the final remap's fields (FUSE) or the
+// new remap vector, the minus-face flux for the dual remap and the PB dual
+// flux.  The sweep order is a template parameter wherever a CellOut is
+// used: a run-time choice between its members makes the compiler merge the
+// two paths' accesses through a selected address, which puts the struct in
+// scratch memory.
+template <class R, int M>
+struct CellOut {
+  FinalOut<R, M> fin;
+  double U[3 * M + 1];
+  double G[M];
+  double DF[M];
+};
+
+template <class R, int M, bool FUSE>
+__device__ __forceinline__ void storeCellOut(const KPT<R>& kp, long c, long fm, const CellOut<R, M>& out, bool withG) {
+  const auto& f = kp.f;
+  if constexpr (FUSE) {
+    finalStore<R, M>(kp, c, out.fin);
+  } else {
+    double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
+#pragma unroll
+    for (int q = 0; q < 3 * M + 1; ++q) Uout[q][c] = out.U[q];
+  }
+  if (withG) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) f.G[k][fm] = out.G[k];
+  }
+}
+
+// Visits the members of out that a launch stores, in a fixed order: the
+// final fields (FUSE) or the remap vector, then G (withG), then DF (withDF).
+template <class R, int M, bool FUSE, class Fn>
+__device__ __forceinline__ void visitCellOut(CellOut<R, M>& out, bool withG, bool withDF, Fn&& fn) {
+  if constexpr (FUSE) {
+    FinalOut<R, M>& fo = out.fin;
+    fn(fo.rho), fn(fo.e), fn(fo.q), fn(fo.p), fn(fo.c), fn(fo.mixte), fn(fo.pure), fn(fo.Mc);
+#pragma unroll
+    for (int k = 0; k < M; ++k) fn(fo.fr[k]), fn(fo.Mk[k]), fn(fo.mu[k]), fn(fo.rk[k]), fn(fo.ek[k]), fn(fo.pk[k]), fn(fo.ck[k]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 3 * M + 1; ++q) fn(out.U[q]);
+  }
+  if (withG) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) fn(out.G[k]);
+  }
+  if (withDF) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) fn(out.DF[k]);
+  }
+}
+
+// 8- and 4-byte members of the final fields
+template <class R, int M>
+constexpr int kFuse8 = 1 + 2 * M + (sizeof(R) == 8 ? 5 + 5 * M : 0);
+template <class R, int M>
+constexpr int kFuse4 = 2 + (sizeof(R) == 4 ? 5 + 5 * M : 0);
+// LDS doubles per staged cell: the larger of the two layouts, each followed
+// by G and DF
+template <class R, int M, bool PP>
+constexpr int kStageBytes = ((kFuse8<R, M> + M + (PP ? M : 0)) * 8 + kFuse4<R, M> * 4) > (3 * M + 1 + M + (PP ? M : 0)) * 8
+                                ? (kFuse8<R, M> + M + (PP ? M : 0)) * 8 + kFuse4<R, M> * 4
+                                : (3 * M + 1 + M + (PP ? M : 0)) * 8;
+template <class R, int M, bool PP>
+constexpr int kStageWords = (kStageBytes<R, M, PP> + 7) / 8;
+
+// LDS staging of the CellOut of a tile of TILE cells, in a buffer of
+// kStageWords * TILE doubles: 8-byte member w of element e at
+// s8[w * TILE + e], then the 4-byte members at s4[w * TILE + e]
+template <int TILE>
+struct OutStage {
+  double* s8;
+  unsigned* s4;
+  template <class R, int M, bool FUSE>
+  __device__ __forceinline__ static OutStage make(double* buf, bool withG, bool withDF) {
+    const int n8 = (FUSE ? kFuse8<R, M> : 3 * M + 1) + (withG ? M : 0) + (withDF ? M : 0);
+    return OutStage{buf, reinterpret_cast<unsigned*>(buf + n8 * TILE)};
+  }
+  template <class R, int M, bool FUSE>
+  __device__ __forceinline__ void put(CellOut<R, M>& out, int e, bool withG, bool withDF) const {
+    int w8 = 0, w4 = 0;
+    visitCellOut<R, M, FUSE>(out, withG, withDF, [&](auto& v) {
+      using V = typename std::remove_reference<decltype(v)>::type;
+      if constexpr (sizeof(V) == 8) s8[(w8++) * TILE + e] = __builtin_bit_cast(double, v);
+      else s4[(w4++) * TILE + e] = __builtin_bit_cast(unsigned, v);
+    });
+  }
+  template <class R, int M, bool FUSE>
+  __device__ __forceinline__ void get(CellOut<R, M>& out, int e, bool withG, bool withDF) const {
+    int w8 = 0, w4 = 0;
+    visitCellOut<R, M, FUSE>(out, withG, withDF, [&](auto& v) {
+      using V = typename std::remove_reference<decltype(v)>::type;
+      if constexpr (sizeof(V) == 8) v = __builtin_bit_cast(V, s8[(w8++) * TILE + e]);
+      else v = __builtin_bit_cast(V, s4[(w4++) * TILE + e]);
+    });
+  }
+};
 
 }  // namespace mhy

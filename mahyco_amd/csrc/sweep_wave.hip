@@ -74,6 +74,57 @@ __device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, kWav
 
 }  // namespace
 
+// Along y a lane's row segment is only 4 cells wide, so the block stages the
+// outputs of its 12 x 4W tile in LDS and then stores them with each thread on
+// one cell, consecutive threads on consecutive cells of a row: every store
+// instruction writes whole 128-byte lines of each field.  Every thread of the
+// block calls this (two barriers).  FUSE: the second sweep (kp.fuseFinal).
+template <class R, int M, bool PP, class T, bool FUSE>
+__device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, long j, long a, long bx, long by,
+                                             int slot, int wave, int lane, bool output, bool hasM, bool hasP,
+                                             double vm, double vp, double Lm, double Lp, const T* phiM,
+                                             const T* phiP, const double* dfv, bool pb, double* stage) {
+  constexpr int NC = 3 * M + 1;
+  const Opts& o = kp.o;
+  const auto& f = kp.f;
+  const long nx = o.nx, ny = o.ny;
+  constexpr int W = kYWaves<R, PP>;
+  constexpr int TW = W * kYCols;    // tile width (columns)
+  constexpr int TILE = kYOut * TW;  // cells per tile, fewer than the block's threads
+  const bool withG = (o.methode == 0 || o.methode == 1);
+  const OutStage<TILE> st = OutStage<TILE>::template make<R, M, FUSE>(stage, withG, pb);
+  if (output) {
+    CellOut<R, M> out;
+    double Unew[NC];
+    sweepUnew<R, M, PP, false, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, out.G);
+    if constexpr (FUSE) {
+      finalCompute<R, M>(kp, c, Unew, out.fin);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NC; ++q) out.U[q] = Unew[q];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) out.DF[k] = dfv[k];
+    st.template put<R, M, FUSE>(out, (slot - kHalo) * TW + wave * kYCols + lane % kYCols, withG, pb);
+  }
+  __syncthreads();
+  const int e = threadIdx.x;
+  if (e < TILE) {
+    const long row = by * kYOut + e / TW, col = bx * TW + e % TW;
+    if (row < ny && col < nx) {
+      CellOut<R, M> out;
+      st.template get<R, M, FUSE>(out, e, withG, pb);
+      const long cc = row * nx + col;
+      storeCellOut<R, M, FUSE>(kp, cc, cc, out, withG);  // the minus face of a y-cell has the cell's index
+      if (pb) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) f.DF[k][cc] = out.DF[k];
+      }
+    }
+  }
+  __syncthreads();  // the batch kernels reuse the tile for their next block
+}
+
 // b: linear block index.  Along x the strips of all rows are numbered
 // row-major and each wave of the block takes the next one; along y a block
 // covers 16 columns x 12 rows, blocks ordered column-group-major per row band.
@@ -90,6 +141,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 
   // cell of this lane: along index a (with na cells), across index t
   long a, t, na, nt;
+  long bx = 0, by = 0;  // y: block column group and row band
   int slot;  // position along the sweep inside the wave strip (0 and last: halo)
   if (XD) {
     const long nseg = (nx + kXOut - 1) / kXOut;  // strips per row
@@ -102,7 +154,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   } else {
     constexpr int W = kYWaves<R, PP>;
     const long gx = (nx + W * kYCols - 1) / (W * kYCols);
-    const long bx = b % gx, by = b / gx;
+    bx = b % gx, by = b / gx;
     slot = lane / kYCols;
     a = by * kYOut - kHalo + slot;
     t = bx * (W * kYCols) + wave * kYCols + (lane % kYCols);
@@ -209,6 +261,9 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   T phiM[NC], phiP[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) phiM[q] = phiP[q] = T(0.0);
+  double dfv[M];  // PB dual flux of an output cell
+#pragma unroll
+  for (int k = 0; k < M; ++k) dfv[k] = 0.0;
   if (!PP) {
     // own limited slope, then the upwind value of each face
     // (UtilesRemap-Impl.h:491-501); back/front cells: x: left/right,
@@ -250,22 +305,48 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     if (pb && output) {
       const double w = (Lm + Lp) * 0.5;
 #pragma unroll
-      for (int k = 0; k < M; ++k) f.DF[k][c] = formed ? (double)Dual[k] * w : 0.0;
+      for (int k = 0; k < M; ++k) dfv[k] = formed ? (double)Dual[k] * w : 0.0;
     }
   }
-  if (!output) return;
-
-  sweepUpdateCell<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP);
+  if constexpr (XD) {
+    if (!output) return;
+    if (pb) {
+#pragma unroll
+      for (int k = 0; k < M; ++k) f.DF[k][c] = dfv[k];
+    }
+    sweepUpdateCell<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP);
+  } else {
+    // one buffer for both sweep orders: kStageWords doubles per tile cell
+    __shared__ double stage[kStageWords<R, M, PP> * kYOut * kYWaves<R, PP> * kYCols];
+    if (kp.fuseFinal)
+      sweepYStaged<R, M, PP, T, true>(kp, c, i, j, a, bx, by, slot, wave, lane, output, hasM, hasP, vm, vp, Lm, Lp,
+                                      phiM, phiP, dfv, pb, stage);
+    else
+      sweepYStaged<R, M, PP, T, false>(kp, c, i, j, a, bx, by, slot, wave, lane, output, hasM, hasP, vm, vp, Lm, Lp,
+                                       phiM, phiP, dfv, pb, stage);
+  }
 }
 
+
+
+// waves per SIMD the y kernels are compiled for: two 8-wave blocks per CU in
+// the mixed form's limited-slope sweeps (at most 128 VGPRs)
+template <class R, bool PP, bool XD>
+constexpr int kMinWaves = (!XD && kMixed<R> && !PP) ? 4 : 1;
+// the same for the ensemble kernels, whose three-material form would spill
 template <class R, int M, bool PP, bool XD>
-__global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>)) void k_sweep_wave(KPT<R> kp) {
+constexpr int kMinWavesBatch = (M < 3) ? kMinWaves<R, PP, XD> : 1;
+
+template <class R, int M, bool PP, bool XD>
+__global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>))
+__attribute__((amdgpu_waves_per_eu(kMinWaves<R, PP, XD>))) void k_sweep_wave(KPT<R> kp) {
   sweepWaveBody<R, M, PP, XD>(kp, XD ? blockIdx.x : xcdBlock());
 }
 
 // all problems of an ensemble in one launch (BatchLaunchT, common.hpp)
 template <class R, int M, bool PP, bool XD, bool ONE>
-__global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>)) void k_sweep_wave_batch(BatchLaunchT<R> bl) {
+__global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>))
+__attribute__((amdgpu_waves_per_eu(kMinWavesBatch<R, M, PP, XD>))) void k_sweep_wave_batch(BatchLaunchT<R> bl) {
   batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) {
     sweepWaveBody<R, M, PP, XD>(kp, local);
   });
