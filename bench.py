@@ -199,6 +199,8 @@ def main():
                     help="fp64 or mixed FP32/FP64 (default: mixed for c4 and c5, fp64 otherwise)")
     ap.add_argument("--report-steps", type=int, default=500,
                     help="steps of the per-field error report against FP64 (mixed form)")
+    ap.add_argument("--projection-order", type=int, choices=[1, 2, 3], default=None,
+                    help="override the workload's remap order (3: third-order faces, FP64 form)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -219,6 +221,11 @@ def main():
     prec = 1 if args.precision == "mixed" else 0
     cfg.precision = prec
     desc = f"{desc}, {PRECISION_NAME[prec]}"
+    traffic_key = args.config
+    if args.projection_order is not None and args.projection_order != cfg.projection_order:
+        cfg.projection_order = args.projection_order
+        desc = f"{desc}, remap order {args.projection_order}"
+        traffic_key = f"{args.config}_o{args.projection_order}"
     ctx = mahyco_amd.Context(cfg)
     cells = cfg.nx * cfg.ny
     ctx.run(args.warmup)
@@ -242,7 +249,7 @@ def main():
     if os.path.exists(args.traffic_json):
         try:
             with open(args.traffic_json) as fh:
-                traffic = json.load(fh).get(args.config, {}).get(dom)
+                traffic = json.load(fh).get(traffic_key, {}).get(dom)
         except (OSError, ValueError):
             traffic = None
     B = algorithmic_bytes_per_cell(cfg.nbmat, prec)

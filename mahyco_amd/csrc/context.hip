@@ -246,15 +246,21 @@ void (*sweepWaveKernel(bool xd, int m, bool pp))(KPT<R>) {
        {k_sweep_wave<R, 3, false, true>, k_sweep_wave<R, 3, true, true>}}};
   return kw[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
-KernelFn sweepKernel(bool wave, bool xd, int m, bool pp) {
+KernelFn sweepKernel(bool wave, bool xd, int m, bool pp, bool o3) {
+  if (o3) {  // third order: per-cell kernel, no PenteBorne (checked at creation)
+    static const KernelFn k3[2][3] = {
+        {k_sweep_cell<1, false, false, true>, k_sweep_cell<2, false, false, true>, k_sweep_cell<3, false, false, true>},
+        {k_sweep_cell<1, false, true, true>, k_sweep_cell<2, false, true, true>, k_sweep_cell<3, false, true, true>}};
+    return k3[xd ? 1 : 0][m - 1];
+  }
   if (wave) return sweepWaveKernel<double>(xd, m, pp);
   static const KernelFn kc[2][3][2] = {
-      {{k_sweep_cell<1, false, false>, k_sweep_cell<1, true, false>},
-       {k_sweep_cell<2, false, false>, k_sweep_cell<2, true, false>},
-       {k_sweep_cell<3, false, false>, k_sweep_cell<3, true, false>}},
-      {{k_sweep_cell<1, false, true>, k_sweep_cell<1, true, true>},
-       {k_sweep_cell<2, false, true>, k_sweep_cell<2, true, true>},
-       {k_sweep_cell<3, false, true>, k_sweep_cell<3, true, true>}}};
+      {{k_sweep_cell<1, false, false, false>, k_sweep_cell<1, true, false, false>},
+       {k_sweep_cell<2, false, false, false>, k_sweep_cell<2, true, false, false>},
+       {k_sweep_cell<3, false, false, false>, k_sweep_cell<3, true, false, false>}},
+      {{k_sweep_cell<1, false, true, false>, k_sweep_cell<1, true, true, false>},
+       {k_sweep_cell<2, false, true, false>, k_sweep_cell<2, true, true, false>},
+       {k_sweep_cell<3, false, true, false>, k_sweep_cell<3, true, true, false>}}};
   return kc[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
 
@@ -289,7 +295,7 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
             [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, retype<float>(kp)); });
     } else {
-      const KernelFn k = sweepKernel(x->sweepWave, xd, x->o.nbmat, x->o.pp == 1);
+      const KernelFn k = sweepKernel(x->sweepWave, xd, x->o.nbmat, x->o.pp == 1, x->o.order == 3);
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, kp); });
     }
   } else if (x->sweepWave && xd) {
@@ -448,11 +454,17 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     return code;
   };
   if (c.schema != MHY_SCHEMA_VNR) return bail(fail(x, MHY_ERR_ARG, "only the VNR scheme is supported"));
-  if (c.projection_order > 2) return bail(fail(x, MHY_ERR_ARG, "projection order 3 is not supported"));
+  if (c.projection_order < 1 || c.projection_order > 3)
+    return bail(fail(x, MHY_ERR_ARG, "projection order must be 1, 2 or 3"));
+  if (c.projection_order == 3 && c.pp == 1)
+    return bail(fail(x, MHY_ERR_ARG, "projection order 3 is not supported with the PenteBorne option"));
+  if (c.projection_order == 3 && c.precision != 0)
+    return bail(fail(x, MHY_ERR_ARG, "projection order 3 runs in the FP64 form only"));
   if (c.precision != 0 && c.precision != 1)
     return bail(fail(x, MHY_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed FP32 / FP64)"));
   x->mixed = (c.precision == 1);
   if (x->mixed) x->sweepWave = true;  // the per-cell stencil sweeps are FP64 only
+  if (c.projection_order == 3) x->sweepWave = false;  // third order: per-cell sweeps
   x->walkY = x->mixed && c.nbmat == 1 && c.pp != 1 && (long)sweepWalkGrid(c.nx, c.ny).x * 4 >= 4096;
   if (const char* wy = std::getenv("MAHYCO_SWEEP_Y")) {
     if (std::string(wy) == "walk") x->walkY = true;
@@ -1092,6 +1104,8 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
     const mhy_config& c = cfgs[i];
     if (c.schema != MHY_SCHEMA_VNR || !c.avec_projection || c.sans_lagrange)
       return batchFail(b, MHY_ERR_ARG, "batch problems must use the VNR Lagrange + remap mode");
+    if (c.projection_order > 2)
+      return batchFail(b, MHY_ERR_ARG, "batch problems use a projection of order 1 or 2");
     if (i > 0 && (c.nbmat != cfgs[0].nbmat || c.pp != cfgs[0].pp || c.precision != cfgs[0].precision))
       return batchFail(b, MHY_ERR_ARG,
                        "batch problems must share the material count, the PenteBorne option and the precision");

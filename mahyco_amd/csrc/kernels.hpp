@@ -31,8 +31,8 @@ template <class R, bool XD, bool ONE>
 __global__ void k_dual_wave_batch(BatchLaunchT<R> bl);
 template <class R, bool XD, bool ONE>
 __global__ void k_dual_batch(BatchLaunchT<R> bl);
-// per-cell stencil sweeps: FP64 form only
-template <int M, bool PP, bool XD>
+// per-cell stencil sweeps: FP64 form only; O3: third-order face values
+template <int M, bool PP, bool XD, bool O3>
 __global__ void k_sweep_cell(KP kp);
 template <class R, int M, bool PP, bool XD>
 __global__ void k_sweep_wave(KPT<R> kp);

@@ -66,9 +66,72 @@ __device__ __forceinline__ void selectSlot(Slot<NC>& d, bool first, const Slot<N
 
 __device__ __forceinline__ bool pureNbh(int fm, int f0, int fp) { return ((f0 >> 8) == 0) && f0 == fp && f0 == fm; }
 
+// Third-order face value (remap/UtilesRemap.cc:197-320, ComputeFluxOrdre3):
+// y[0..5] and h[0..5] are the cell means and widths of the three cells on
+// the minus side (farthest first) and the three on the plus side, vdt the
+// face normal velocity times dt.  Each interior face gets the parabola
+// edge value ys from the two-cell gradients, the upwind cell's parabola is
+// averaged over the swept length, and the mean is clamped to the
+// intersection of [ym, yp] and the interval the upwind slope allows (an
+// empty intersection clamps to 0).  At vdt = 0 the extrapolated bound is
+// +-inf or NaN; min and max are the reference's (b < a ? b : a,
+// a < b ? b : a), so a NaN bound collapses to the cell mean as there.
+// Arithmetic order follows the reference.  For vdt < 0 the parabola term
+// enters with the reference's sign (Remap::evaluate_fp).
+__device__ __forceinline__ double o3Grad(double hm, double h0, double hp, double ym, double y0, double yp) {
+  return h0 / (hm + h0 + hp) * ((2. * hm + h0) / (h0 + hp) * (yp - y0) + (h0 + 2. * hp) / (hm + h0) * (y0 - ym));
+}
+
+__device__ __forceinline__ double o3Edge(double hmm, double hm, double hp, double hpp, double ymm, double ym, double yp,
+                                         double ypp, double gm, double gp) {
+  const double t1 = (2. * hp * hm) / (hm + hp) * ((hmm + hm) / (2. * hm + hp) - (hpp + hp) / (2. * hp + hm)) * (yp - ym);
+  const double t2 = -hm * (hmm + hm) / (2. * hm + hp) * gp + hp * (hp + hpp) / (hm + 2. * hp) * gm;
+  return ym + hm / (hm + hp) * (yp - ym) + 1. / (hmm + hm + hp + hpp) * (t1 + t2);
+}
+
+__device__ __forceinline__ double fluxOrdre3(const double* y, const double* h, double vdt) {
+  const double gmm = o3Grad(h[0], h[1], h[2], y[0], y[1], y[2]);
+  const double gm = o3Grad(h[1], h[2], h[3], y[1], y[2], y[3]);
+  const double gp = o3Grad(h[2], h[3], h[4], y[2], y[3], y[4]);
+  const double gpp = o3Grad(h[3], h[4], h[5], y[3], y[4], y[5]);
+  const double eM = o3Edge(h[0], h[1], h[2], h[3], y[0], y[1], y[2], y[3], gmm, gm);  // left edge of the minus cell
+  const double e0 = o3Edge(h[1], h[2], h[3], h[4], y[1], y[2], y[3], y[4], gm, gp);   // the face
+  const double eP = o3Edge(h[2], h[3], h[4], h[5], y[2], y[3], y[4], y[5], gp, gpp);  // right edge of the plus cell
+  const double ym = y[2], yp = y[3];
+  double flux;
+  if (vdt >= 0.) {
+    const double x = vdt, dx = h[2];
+    const double six = 6. * (ym - 0.5 * (eM + e0));
+    flux = e0 - 0.5 * x / dx * ((e0 - eM) - (1. - 2. / 3. * x / dx) * six);
+  } else {
+    const double x = -vdt, dx = h[3];
+    const double six = 6. * (yp - 0.5 * (e0 + eP));
+    flux = e0 + 0.5 * x / dx * ((eP - e0) - (1. - 2. / 3. * x / dx) * six);
+  }
+  const double num = vdt / h[2], nup = vdt / h[3];
+  const double i1lo = smin(ym, yp), i1hi = smax(ym, yp);
+  double i2lo, i2hi;
+  if (vdt >= 0.) {
+    const double ext = ym + (1. - num) / num * (ym - y[1]);
+    i2lo = smin(ym, ext), i2hi = smax(ym, ext);
+  } else {
+    const double ext = yp - (1. + nup) / nup * (yp - y[4]);
+    i2lo = smin(yp, ext), i2hi = smax(yp, ext);
+  }
+  double lo, hi;
+  if ((i1hi < i2lo) || (i2hi < i1lo)) lo = 0., hi = 0.;
+  else lo = smax(i1lo, i2lo), hi = smin(i1hi, i2hi);
+  if (flux < lo) flux = lo;
+  if (flux > hi) flux = hi;
+  return flux;
+}
+
 }  // namespace
 
-template <int M, bool PP, bool XD>
+// O3: third-order face values (projectionOrder 3, no PenteBorne): cells
+// a-3 .. a+3 along the sweep, each neighbour lookup past the domain edge
+// returning the cell itself (remap/UtilesMesh.cc:5-51)
+template <int M, bool PP, bool XD, bool O3>
 __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
   constexpr int NC = 3 * M + 1;
   const Opts& o = kp.o;
@@ -92,7 +155,7 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
   for (int d = 0; d < 5; ++d) {
     const long b = a + d - 2;
     in[d] = (b >= 0 && b < na);
-    if (!in[d]) continue;
+    if (O3 || !in[d]) continue;  // the third-order faces load their own stencil
     const long cd = c + (d - 2) * stride;
     double u[NC];
     const int nload = PP ? 3 * M : NC;
@@ -141,7 +204,29 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
   double phiM[NC], phiP[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) phiM[q] = phiP[q] = 0.0;
-  if (!PP) {
+  if constexpr (O3) {
+    // PhaseRemap1.cc:365-389, PhaseRemap2.cc:359-385: slot k holds the cell
+    // a-3+k, clamped to the line; the minus face uses slots 0..5, the plus
+    // face slots 1..6
+    long id[7];
+    double V[7], H[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const long b = min(max(a - 3 + k, 0L), na - 1);
+      id[k] = c + (b - a) * stride;
+      V[k] = f.VL[id[k]];
+      H[k] = widthC<XD>(kp, dt1, XD ? j : b, XD ? b : i);
+    }
+    const double vdtM = vm * dtn, vdtP = vp * dtn;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      double y[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) y[k] = Uin[q][id[k]] / V[k];
+      if (hasM) phiM[q] = fluxOrdre3(y, H, vdtM);
+      if (hasP) phiP[q] = fluxOrdre3(y + 1, H + 1, vdtP);
+    }
+  } else if (!PP) {
     // upwind value (UtilesRemap-Impl.h:491-501) of a face between slots
     // lo and lo+1; back/front cells: x: left/right, y: above/below
     auto faceValue = [&](long b, const Slot<NC>& sl0, const Slot<NC>& sl1, const Slot<NC>& sl2, const Slot<NC>& sl3,
@@ -249,15 +334,18 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
   }
 }
 
-#define MHY_SWEEP_CELL_INST(M, PP)                       \
-  template __global__ void k_sweep_cell<M, PP, true>(KP); \
-  template __global__ void k_sweep_cell<M, PP, false>(KP);
-MHY_SWEEP_CELL_INST(1, false)
-MHY_SWEEP_CELL_INST(1, true)
-MHY_SWEEP_CELL_INST(2, false)
-MHY_SWEEP_CELL_INST(2, true)
-MHY_SWEEP_CELL_INST(3, false)
-MHY_SWEEP_CELL_INST(3, true)
+#define MHY_SWEEP_CELL_INST(M, PP, O3)                       \
+  template __global__ void k_sweep_cell<M, PP, true, O3>(KP); \
+  template __global__ void k_sweep_cell<M, PP, false, O3>(KP);
+MHY_SWEEP_CELL_INST(1, false, false)
+MHY_SWEEP_CELL_INST(1, true, false)
+MHY_SWEEP_CELL_INST(2, false, false)
+MHY_SWEEP_CELL_INST(2, true, false)
+MHY_SWEEP_CELL_INST(3, false, false)
+MHY_SWEEP_CELL_INST(3, true, false)
+MHY_SWEEP_CELL_INST(1, false, true)
+MHY_SWEEP_CELL_INST(2, false, true)
+MHY_SWEEP_CELL_INST(3, false, true)
 #undef MHY_SWEEP_CELL_INST
 
 }  // namespace mhy

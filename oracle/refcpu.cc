@@ -2134,6 +2134,10 @@ double ref_probe_gradient(ref_ctx* r, int lim, double gp, double gm, double phi,
   return r->computeAndLimitGradPhi(lim, vgp, vgm, vphi, vphip, vphim, h0, hp, hm)[0];
 }
 
+double ref_probe_flux_o3(ref_ctx* r, const double* y, const double* h, double vdt) {
+  return r->ComputeFluxOrdre3(y[0], y[1], y[2], y[3], y[4], y[5], h[0], h[1], h[2], h[3], h[4], h[5], vdt);
+}
+
 double ref_probe_int2y(double X, double x0, double y0, double x1, double y1) {
   return ref_ctx::INT2Y(X, x0, y0, x1, y1);
 }

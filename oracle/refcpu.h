@@ -41,6 +41,11 @@ double ref_probe_int2y(double X, double x0, double y0, double x1, double y1);
 double ref_probe_flux_pp(ref_ctx* r, int lim, double grad, double phi, double phip, double phim, double h0, double hp,
                          double hm, double fnv, double dt, int type);
 
+/* The third-order face value of one component (remap/UtilesRemap.cc:197-320):
+ * y, h = means and widths of the cells c-3 .. c+2 around the face, vdt = face
+ * normal velocity times dt. */
+double ref_probe_flux_o3(ref_ctx* r, const double* y, const double* h, double vdt);
+
 int ref_get_field(ref_ctx* r, int field, double* out, long long n);
 int ref_get_time_state(ref_ctx* r, mhy_time_state* ts);
 int ref_set_threads(int nthreads);

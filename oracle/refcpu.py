@@ -52,6 +52,8 @@ def lib():
         L.ref_probe_int2y.restype = d
         L.ref_probe_flux_pp.argtypes = [ctypes.c_void_p, ctypes.c_int] + [d] * 9 + [ctypes.c_int]
         L.ref_probe_flux_pp.restype = d
+        L.ref_probe_flux_o3.argtypes = [ctypes.c_void_p, ctypes.POINTER(d), ctypes.POINTER(d), d]
+        L.ref_probe_flux_o3.restype = d
         _lib = L
     return _lib
 
@@ -121,6 +123,13 @@ class Oracle:
     def probe_flux_pp(self, lim, grad, phi, phip, phim, h0, hp, hm, fnv, dt, face):
         """PenteBorne swept flux of one component; face 0 = back, 1 = front."""
         return lib().ref_probe_flux_pp(self.h, lim, grad, phi, phip, phim, h0, hp, hm, fnv, dt, face)
+
+    def probe_flux_o3(self, y, h, vdt):
+        """Third-order face value of one component (remap/UtilesRemap.cc:197-320);
+        y, h: the six cell means and widths around the face, minus side first."""
+        ya = (ctypes.c_double * 6)(*[float(v) for v in y])
+        ha = (ctypes.c_double * 6)(*[float(v) for v in h])
+        return lib().ref_probe_flux_o3(self.h, ya, ha, float(vdt))
 
     def get(self, name, k=0):
         fid, n = self.field_id(name, k)

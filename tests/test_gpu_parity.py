@@ -77,6 +77,17 @@ def extension_cases():
     c = configs.riemann(32, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
     c.limiter = abi.LIM["superBeeG"]
     out["riemann_superBeeG_main"] = c
+    # third-order remap (PhaseRemap1.cc:365-389): two materials along x,
+    # one material in both directions, three materials on a 2-D layout
+    c = configs.riemann(40, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.67)
+    c.projection_order = 3
+    out["riemann_order3"] = c
+    c = _small(configs.point_explosion(), 32)
+    c.projection_order = 3
+    out["C3_point_explosion_order3"] = c
+    c = _small(configs.triple_layout(), 48, 16)
+    c.projection_order = 3
+    out["C4_triple_layout_order3"] = c
     c = configs.riemann(32, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
     c.pseudo_centree = 1
     c.abort_on_energy = 1
@@ -117,6 +128,30 @@ def error_cases():
 
 
 ERR = error_cases()
+
+
+def rejected_cases():
+    out = {}
+    c = configs.riemann(16, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
+    c.projection_order, c.pp = 3, 1
+    out["order3_pentborne"] = c
+    c = configs.riemann(16, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
+    c.projection_order, c.precision = 3, 1
+    out["order3_mixed"] = c
+    c = configs.riemann(16, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
+    c.projection_order = 4
+    out["order4"] = c
+    return out
+
+
+REJ = rejected_cases()
+
+
+@pytest.mark.parametrize("name", sorted(REJ))
+def test_unsupported_options_are_rejected(name):
+    with pytest.raises(mahyco_amd.MahycoError) as e:
+        mahyco_amd.Context(REJ[name])
+    assert f"({abi.MHY_ERR_ARG})" in str(e.value)
 
 
 @pytest.mark.parametrize("name", sorted(ERR))

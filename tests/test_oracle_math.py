@@ -237,6 +237,122 @@ def _triple(**opts):
     return c
 
 
+# ---- third-order face values (remap/UtilesRemap.cc:197-320)
+#
+# The edge values are Colella & Woodward's (J. Comput. Phys. 54, 1984,
+# eq. 1.6-1.7) quartic fit of the cumulative integral: exact for the cell
+# means of any cubic, on any widths.  The swept mean of the upwind
+# parabola is then exact for a quadratic.  Face between cell 2 and cell 3.
+
+
+def _o3_grid(rng, uniform):
+    h = np.ones(6) if uniform else rng.uniform(0.5, 2.0, 6)
+    edges = 1.0 + np.concatenate([[0.0], np.cumsum(h)])
+    return h, edges
+
+
+def _o3_means(P, edges):
+    """Cell means of the polynomial with antiderivative P."""
+    return np.array([(P(edges[k + 1]) - P(edges[k])) / (edges[k + 1] - edges[k]) for k in range(6)])
+
+
+def _o3_clamp_ok(y, h, vdt, value):
+    """value lies in [min(ym, yp), max(ym, yp)] and in the interval the
+    upwind slope allows (the clamp the formula applies)."""
+    ym, yp = y[2], y[3]
+    if vdt >= 0:
+        nu = vdt / h[2]
+        ext = ym + (1 - nu) / nu * (ym - y[1])
+        lo2, hi2 = min(ym, ext), max(ym, ext)
+    else:
+        nu = vdt / h[3]
+        ext = yp - (1 + nu) / nu * (yp - y[4])
+        lo2, hi2 = min(yp, ext), max(yp, ext)
+    lo, hi = max(min(ym, yp), lo2), min(max(ym, yp), hi2)
+    return lo <= value <= hi
+
+
+@pytest.mark.parametrize("uniform", [True, False], ids=["uniform", "nonuniform"])
+def test_order3_flux_is_exact_on_quadratics(orc, uniform):
+    """Positive velocity: the mean of a quadratic over the swept part of
+    the upwind cell, on uniform and random widths."""
+    rng = np.random.default_rng(31 + uniform)
+    checked = 0
+    for _ in range(200):
+        h, edges = _o3_grid(rng, uniform)
+        A, B, C = rng.uniform(0.05, 0.5), rng.uniform(1.0, 3.0), rng.uniform(-1, 1)
+        Q = lambda x: A * x ** 3 / 3 + B * x ** 2 / 2 + C * x  # noqa: E731
+        y = _o3_means(Q, edges)
+        xf, vdt = edges[3], rng.uniform(0.05, 0.4) * h[2]
+        exact = (Q(xf) - Q(xf - vdt)) / vdt
+        if not _o3_clamp_ok(y, h, vdt, exact):
+            continue
+        checked += 1
+        got = orc.probe_flux_o3(y, h, vdt)
+        assert got == pytest.approx(exact, rel=1e-12, abs=1e-12)
+    assert checked > 150
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0], ids=["positive", "negative"])
+def test_order3_face_value_is_exact_for_cubics_on_any_widths(orc, sign):
+    """As the swept length goes to 0, the face value is the cubic's value
+    at the face, from either side: it pins both gradients and the edge
+    formula on unequal widths."""
+    rng = np.random.default_rng(7 if sign > 0 else 8)
+    for _ in range(100):
+        h, edges = _o3_grid(rng, False)
+        a3, a2, a1 = rng.uniform(0.01, 0.1), rng.uniform(0.0, 0.3), rng.uniform(1.0, 3.0)
+        P = lambda x: a3 * x ** 4 / 4 + a2 * x ** 3 / 3 + a1 * x ** 2 / 2  # noqa: E731
+        q = lambda x: a3 * x ** 3 + a2 * x ** 2 + a1 * x  # noqa: E731
+        y = _o3_means(P, edges)
+        got = orc.probe_flux_o3(y, h, sign * 1e-10 * h[2])
+        assert got == pytest.approx(q(edges[3]), rel=1e-8)
+
+
+def test_order3_flux_is_exact_on_linear_data_both_ways(orc):
+    rng = np.random.default_rng(11)
+    for _ in range(100):
+        h, edges = _o3_grid(rng, False)
+        B, C = rng.uniform(-3, 3), rng.uniform(-1, 1)
+        Q = lambda x: B * x ** 2 / 2 + C * x  # noqa: E731
+        y = _o3_means(Q, edges)
+        xf = edges[3]
+        for vdt in (rng.uniform(0.05, 0.4) * h[2], -rng.uniform(0.05, 0.4) * h[3]):
+            exact = B * (xf - vdt / 2) + C
+            assert orc.probe_flux_o3(y, h, vdt) == pytest.approx(exact, rel=1e-12, abs=1e-12)
+
+
+def test_order3_negative_velocity_parabola_term_sign(orc):
+    """For a negative velocity the reference adds the parabola term with
+    the sign opposite to Colella & Woodward's eq. 1.12 (Remap::evaluate_fp):
+    on the means of q = A x^2 + B x the value is the exact swept mean plus
+    A x (dx - 2x/3), x = |v dt|, dx the upwind width.  Kept as is."""
+    rng = np.random.default_rng(12)
+    checked = 0
+    for _ in range(200):
+        h, edges = _o3_grid(rng, False)
+        A, B = -rng.uniform(0.01, 0.05), rng.uniform(2.0, 3.0)  # concave, increasing on [1, 13]
+        Q = lambda x: A * x ** 3 / 3 + B * x ** 2 / 2  # noqa: E731
+        y = _o3_means(Q, edges)
+        xf, x = edges[3], rng.uniform(0.05, 0.4) * h[3]
+        want = (Q(xf + x) - Q(xf)) / x + A * x * (h[3] - 2 * x / 3)
+        if not _o3_clamp_ok(y, h, -x, want):
+            continue
+        checked += 1
+        assert orc.probe_flux_o3(y, h, -x) == pytest.approx(want, rel=1e-11, abs=1e-12)
+    assert checked > 150
+
+
+def test_order3_flux_stays_between_the_face_neighbours(orc):
+    rng = np.random.default_rng(13)
+    for _ in range(2000):
+        h = rng.uniform(0.3, 3.0, 6)
+        y = rng.uniform(-2.0, 2.0, 6)
+        vdt = rng.uniform(-0.9, 0.9) * (h[2] if rng.uniform() < 0.5 else h[3])
+        got = orc.probe_flux_o3(y, h, vdt)
+        assert min(y[2], y[3]) <= got <= max(y[2], y[3])
+
+
 RUNS = {
     "minmodG": dict(limiter=LIM["minmodG"]),
     "superBeeG": dict(limiter=LIM["superBeeG"]),
@@ -249,6 +365,7 @@ RUNS = {
     "pp_A2": dict(limiter=LIM["superBee"], pp=1, methode_flux_masse=1),
     "pp_PB": dict(limiter=LIM["superBee"], pp=1, methode_flux_masse=2),
     "order1": dict(projection_order=1),
+    "order3": dict(projection_order=3),
 }
 
 
@@ -279,7 +396,7 @@ def test_run_conserves_mass_and_keeps_fractions(name):
     np.testing.assert_allclose(fr.sum(axis=0), 1.0, rtol=0, atol=1e-13)
 
 
-@pytest.mark.parametrize("name", ["minmodG", "pp_mixte", "pp_PB", "vanLeer"])
+@pytest.mark.parametrize("name", ["minmodG", "pp_mixte", "pp_PB", "vanLeer", "order3"])
 def test_uniform_state_is_left_unchanged(name):
     """Two identical materials at rest: every field keeps its initial value."""
     cfg = _small_riemann(**RUNS[name])
@@ -303,8 +420,9 @@ def _fixture(name):
 
 @pytest.mark.parametrize("opts", [dict(limiter=LIM["minmodG"]), dict(limiter=LIM["vanLeerG"]),
                                   dict(pp=1, pp_mixte=1, limiter=LIM["superBee"], limiter_pure=LIM["ultrabeeG"]),
-                                  dict(pp=1, methode_flux_masse=2, limiter=LIM["superBee"])],
-                         ids=["minmodG", "vanLeerG", "pp_mixte", "pp_PB"])
+                                  dict(pp=1, methode_flux_masse=2, limiter=LIM["superBee"]),
+                                  dict(projection_order=3)],
+                         ids=["minmodG", "vanLeerG", "pp_mixte", "pp_PB", "order3"])
 def test_x_and_y_runs_mirror(opts):
     """The two-material Sod problem along x on 200x5 and along y on 5x200,
     with options no fixture uses: the y run is the transposed x run."""
