@@ -11,7 +11,7 @@ three-material layout in the mixed FP32 / FP64 form with a per-field error
 report against the FP64 form after the run; --config c5 runs the 512-problem
 ensemble batched (mixed form by default) against the plain FP64 form.
 
-Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: the library runs on one GPU.
 """
 import argparse
 import json
@@ -208,7 +208,7 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     import mahyco_amd

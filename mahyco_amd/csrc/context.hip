@@ -246,21 +246,15 @@ void (*sweepWaveKernel(bool xd, int m, bool pp))(KPT<R>) {
        {k_sweep_wave<R, 3, false, true>, k_sweep_wave<R, 3, true, true>}}};
   return kw[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
-KernelFn sweepKernel(bool wave, bool xd, int m, bool pp, bool o3) {
-  if (o3) {  // third order: per-cell kernel, no PenteBorne (checked at creation)
-    static const KernelFn k3[2][3] = {
-        {k_sweep_cell<1, false, false, true>, k_sweep_cell<2, false, false, true>, k_sweep_cell<3, false, false, true>},
-        {k_sweep_cell<1, false, true, true>, k_sweep_cell<2, false, true, true>, k_sweep_cell<3, false, true, true>}};
-    return k3[xd ? 1 : 0][m - 1];
-  }
+KernelFn sweepKernel(bool wave, bool xd, int m, bool pp) {
   if (wave) return sweepWaveKernel<double>(xd, m, pp);
   static const KernelFn kc[2][3][2] = {
-      {{k_sweep_cell<1, false, false, false>, k_sweep_cell<1, true, false, false>},
-       {k_sweep_cell<2, false, false, false>, k_sweep_cell<2, true, false, false>},
-       {k_sweep_cell<3, false, false, false>, k_sweep_cell<3, true, false, false>}},
-      {{k_sweep_cell<1, false, true, false>, k_sweep_cell<1, true, true, false>},
-       {k_sweep_cell<2, false, true, false>, k_sweep_cell<2, true, true, false>},
-       {k_sweep_cell<3, false, true, false>, k_sweep_cell<3, true, true, false>}}};
+      {{k_sweep_cell<1, false, false>, k_sweep_cell<1, true, false>},
+       {k_sweep_cell<2, false, false>, k_sweep_cell<2, true, false>},
+       {k_sweep_cell<3, false, false>, k_sweep_cell<3, true, false>}},
+      {{k_sweep_cell<1, false, true>, k_sweep_cell<1, true, true>},
+       {k_sweep_cell<2, false, true>, k_sweep_cell<2, true, true>},
+       {k_sweep_cell<3, false, true>, k_sweep_cell<3, true, true>}}};
   return kc[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
 
@@ -277,9 +271,13 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
   kp.xdir = xd ? 1 : 0;
   if (stage == MHY_STAGE_CELLS) {
-    const dim3 grid = x->sweepWave ? sweepWaveGrid(xd, x->o.nx, x->o.ny, x->mixed, x->o.pp == 1) : gridFor(x->ncell);
-    const dim3 block(x->sweepWave ? sweepWaveBlock(xd, x->mixed, x->o.pp == 1) : kBlock);
-    if (!xd && x->sweepWave && x->walkY) {
+    // third order: LDS-tile kernel (the nodal stage keeps its own choice)
+    const bool o3 = (x->o.order == 3), wave = x->sweepWave && !o3;
+    const dim3 grid = wave ? sweepWaveGrid(xd, x->o.nx, x->o.ny, x->mixed, x->o.pp == 1)
+                      : o3 ? sweepO3Grid(xd, x->o.nx, x->o.ny)
+                           : gridFor(x->ncell);
+    const dim3 block(wave ? sweepWaveBlock(xd, x->mixed, x->o.pp == 1) : kBlock);
+    if (!xd && wave && x->walkY) {
       static const KernelFn kd[3][2] = {{k_sweep_walk<double, 1, false>, k_sweep_walk<double, 1, true>},
                                         {k_sweep_walk<double, 2, false>, k_sweep_walk<double, 2, true>},
                                         {k_sweep_walk<double, 3, false>, k_sweep_walk<double, 3, true>}};
@@ -290,12 +288,21 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
       timed(x, K_SWEEP_Y, [&] {
         launchR(x, kd[mi][pi], kf[mi][pi], sweepWalkGrid(x->o.nx, x->o.ny), dim3(kBlock), kp);
       });
+    } else if (o3) {  // third order: LDS-tile kernel, no PenteBorne (checked at creation)
+      static const KernelFn kd[2][3] = {
+          {k_sweep_o3<double, 1, false>, k_sweep_o3<double, 2, false>, k_sweep_o3<double, 3, false>},
+          {k_sweep_o3<double, 1, true>, k_sweep_o3<double, 2, true>, k_sweep_o3<double, 3, true>}};
+      static const KernelFnF kf[2][3] = {
+          {k_sweep_o3<float, 1, false>, k_sweep_o3<float, 2, false>, k_sweep_o3<float, 3, false>},
+          {k_sweep_o3<float, 1, true>, k_sweep_o3<float, 2, true>, k_sweep_o3<float, 3, true>}};
+      const int di = xd ? 1 : 0, mi = x->o.nbmat - 1;
+      timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { launchR(x, kd[di][mi], kf[di][mi], grid, block, kp); });
     } else if (x->mixed) {
       const KernelFnF k = sweepWaveKernel<float>(xd, x->o.nbmat, x->o.pp == 1);
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
             [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, retype<float>(kp)); });
     } else {
-      const KernelFn k = sweepKernel(x->sweepWave, xd, x->o.nbmat, x->o.pp == 1, x->o.order == 3);
+      const KernelFn k = sweepKernel(wave, xd, x->o.nbmat, x->o.pp == 1);
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, kp); });
     }
   } else if (x->sweepWave && xd) {
@@ -458,13 +465,10 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     return bail(fail(x, MHY_ERR_ARG, "projection order must be 1, 2 or 3"));
   if (c.projection_order == 3 && c.pp == 1)
     return bail(fail(x, MHY_ERR_ARG, "projection order 3 is not supported with the PenteBorne option"));
-  if (c.projection_order == 3 && c.precision != 0)
-    return bail(fail(x, MHY_ERR_ARG, "projection order 3 runs in the FP64 form only"));
   if (c.precision != 0 && c.precision != 1)
     return bail(fail(x, MHY_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed FP32 / FP64)"));
   x->mixed = (c.precision == 1);
   if (x->mixed) x->sweepWave = true;  // the per-cell stencil sweeps are FP64 only
-  if (c.projection_order == 3) x->sweepWave = false;  // third order: per-cell sweeps
   x->walkY = x->mixed && c.nbmat == 1 && c.pp != 1 && (long)sweepWalkGrid(c.nx, c.ny).x * 4 >= 4096;
   if (const char* wy = std::getenv("MAHYCO_SWEEP_Y")) {
     if (std::string(wy) == "walk") x->walkY = true;

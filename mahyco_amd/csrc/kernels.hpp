@@ -31,9 +31,13 @@ template <class R, bool XD, bool ONE>
 __global__ void k_dual_wave_batch(BatchLaunchT<R> bl);
 template <class R, bool XD, bool ONE>
 __global__ void k_dual_batch(BatchLaunchT<R> bl);
-// per-cell stencil sweeps: FP64 form only; O3: third-order face values
-template <int M, bool PP, bool XD, bool O3>
+// per-cell stencil sweeps: FP64 form only
+template <int M, bool PP, bool XD>
 __global__ void k_sweep_cell(KP kp);
+// third-order sweep on LDS tiles (256-thread blocks)
+template <class R, int M, bool XD>
+__global__ void k_sweep_o3(KPT<R> kp);
+dim3 sweepO3Grid(bool xd, long nx, long ny);
 template <class R, int M, bool PP, bool XD>
 __global__ void k_sweep_wave(KPT<R> kp);
 dim3 sweepWaveGrid(bool xd, long nx, long ny, bool mixed, bool pp);

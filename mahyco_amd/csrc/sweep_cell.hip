@@ -9,9 +9,13 @@
 // stencil values live in registers.  Operation order follows the reference,
 // so FP64 results are bitwise those of the CPU path.  The face fluxes are
 // also written per face for the nodal (dual) momentum remap.
+//
+// The file also holds the third-order sweep (projectionOrder 3), which reads
+// cells a-3 .. a+3 from an LDS tile (k_sweep_o3).
 #include "final_math.hpp"
 #include "kernels.hpp"
 #include "sweep_math.hpp"
+#include "sweep_update.hpp"
 
 namespace mhy {
 
@@ -66,56 +70,87 @@ __device__ __forceinline__ void selectSlot(Slot<NC>& d, bool first, const Slot<N
 
 __device__ __forceinline__ bool pureNbh(int fm, int f0, int fp) { return ((f0 >> 8) == 0) && f0 == fp && f0 == fm; }
 
-// Third-order face value (remap/UtilesRemap.cc:197-320, ComputeFluxOrdre3):
-// y[0..5] and h[0..5] are the cell means and widths of the three cells on
-// the minus side (farthest first) and the three on the plus side, vdt the
-// face normal velocity times dt.  Each interior face gets the parabola
-// edge value ys from the two-cell gradients, the upwind cell's parabola is
-// averaged over the swept length, and the mean is clamped to the
-// intersection of [ym, yp] and the interval the upwind slope allows (an
-// empty intersection clamps to 0).  At vdt = 0 the extrapolated bound is
-// +-inf or NaN; min and max are the reference's (b < a ? b : a,
-// a < b ? b : a), so a NaN bound collapses to the cell mean as there.
-// Arithmetic order follows the reference.  For vdt < 0 the parabola term
-// enters with the reference's sign (Remap::evaluate_fp).
-__device__ __forceinline__ double o3Grad(double hm, double h0, double hp, double ym, double y0, double yp) {
-  return h0 / (hm + h0 + hp) * ((2. * hm + h0) / (h0 + hp) * (yp - y0) + (h0 + 2. * hp) / (hm + h0) * (y0 - ym));
+// Third-order face value (remap/UtilesRemap.cc:197-320, ComputeFluxOrdre3).
+// Around a face, y[0..5] and h[0..5] are the cell means and widths of the
+// three cells on the minus side (farthest first) and the three on the plus
+// side, vdt the face normal velocity times dt.  Each interior edge gets the
+// value of Colella & Woodward's quartic fit from the two-cell gradients,
+// the upwind cell's parabola is averaged over the swept length, and the
+// mean is clamped to the intersection of [ym, yp] and the interval the
+// upwind slope allows (an empty intersection clamps to 0).  At vdt = 0 the
+// extrapolated bound is +-inf or NaN; min and max are the reference's
+// (b < a ? b : a, a < b ? b : a), so a NaN bound collapses to the cell mean
+// as there.  For vdt < 0 the parabola term enters with the reference's sign
+// (Remap::evaluate_fp).
+//
+// Every quotient of widths alone is formed once per cell and shared by the
+// components: in the reference's expressions those quotients are the
+// leftmost operands of left-to-right products, so taking them first changes
+// no rounding.
+
+// gradient of the cell of width h0 between widths hm and hp
+// (Remap::evaluate_grad): g = c0 * (c1 * (yp - y0) + c2 * (y0 - ym))
+struct O3Grad {
+  double c0, c1, c2;
+};
+__device__ __forceinline__ O3Grad o3GradCoef(double hm, double h0, double hp) {
+  return {h0 / (hm + h0 + hp), (2. * hm + h0) / (h0 + hp), (h0 + 2. * hp) / (hm + h0)};
+}
+__device__ __forceinline__ double o3Grad(const O3Grad& k, double ym, double y0, double yp) {
+  return k.c0 * (k.c1 * (yp - y0) + k.c2 * (y0 - ym));
 }
 
-__device__ __forceinline__ double o3Edge(double hmm, double hm, double hp, double hpp, double ymm, double ym, double yp,
-                                         double ypp, double gm, double gp) {
-  const double t1 = (2. * hp * hm) / (hm + hp) * ((hmm + hm) / (2. * hm + hp) - (hpp + hp) / (2. * hp + hm)) * (yp - ym);
-  const double t2 = -hm * (hmm + hm) / (2. * hm + hp) * gp + hp * (hp + hpp) / (hm + 2. * hp) * gm;
-  return ym + hm / (hm + hp) * (yp - ym) + 1. / (hmm + hm + hp + hpp) * (t1 + t2);
+// value at the edge between the cells of widths hm and hp
+// (Remap::evaluate_ystar)
+struct O3Edge {
+  double t1, am, ap, w, s;
+};
+__device__ __forceinline__ O3Edge o3EdgeCoef(double hmm, double hm, double hp, double hpp) {
+  return {(2. * hp * hm) / (hm + hp) * ((hmm + hm) / (2. * hm + hp) - (hpp + hp) / (2. * hp + hm)),
+          -hm * (hmm + hm) / (2. * hm + hp), hp * (hp + hpp) / (hm + 2. * hp), hm / (hm + hp),
+          1. / (hmm + hm + hp + hpp)};
+}
+__device__ __forceinline__ double o3Edge(const O3Edge& k, double ym, double yp, double gm, double gp) {
+  const double t1 = k.t1 * (yp - ym);
+  const double t2 = k.am * gp + k.ap * gm;
+  return ym + k.w * (yp - ym) + k.s * (t1 + t2);
 }
 
-__device__ __forceinline__ double fluxOrdre3(const double* y, const double* h, double vdt) {
-  const double gmm = o3Grad(h[0], h[1], h[2], y[0], y[1], y[2]);
-  const double gm = o3Grad(h[1], h[2], h[3], y[1], y[2], y[3]);
-  const double gp = o3Grad(h[2], h[3], h[4], y[2], y[3], y[4]);
-  const double gpp = o3Grad(h[3], h[4], h[5], y[3], y[4], y[5]);
-  const double eM = o3Edge(h[0], h[1], h[2], h[3], y[0], y[1], y[2], y[3], gmm, gm);  // left edge of the minus cell
-  const double e0 = o3Edge(h[1], h[2], h[3], h[4], y[1], y[2], y[3], y[4], gm, gp);   // the face
-  const double eP = o3Edge(h[2], h[3], h[4], h[5], y[2], y[3], y[4], y[5], gp, gpp);  // right edge of the plus cell
+// the swept-mean factors of a face: x = |vdt|, dx = the upwind width
+struct O3Sweep {
+  double p1, p2, r;
+  bool pos;
+};
+__device__ __forceinline__ O3Sweep o3SweepCoef(double vdt, double hm, double hp) {
+  O3Sweep k;
+  k.pos = (vdt >= 0.);
+  const double x = k.pos ? vdt : -vdt, dx = k.pos ? hm : hp;
+  k.p1 = 0.5 * x / dx;
+  k.p2 = 1. - 2. / 3. * x / dx;
+  const double nu = k.pos ? vdt / hm : vdt / hp;
+  k.r = k.pos ? (1. - nu) / nu : (1. + nu) / nu;
+  return k;
+}
+
+// y[0..5]: the cell means around the face; eM, e0, eP: the left edge of
+// the minus cell, the face, the right edge of the plus cell
+__device__ __forceinline__ double o3Flux(const O3Sweep& k, const double* y, double eM, double e0, double eP) {
   const double ym = y[2], yp = y[3];
   double flux;
-  if (vdt >= 0.) {
-    const double x = vdt, dx = h[2];
+  if (k.pos) {
     const double six = 6. * (ym - 0.5 * (eM + e0));
-    flux = e0 - 0.5 * x / dx * ((e0 - eM) - (1. - 2. / 3. * x / dx) * six);
+    flux = e0 - k.p1 * ((e0 - eM) - k.p2 * six);
   } else {
-    const double x = -vdt, dx = h[3];
     const double six = 6. * (yp - 0.5 * (e0 + eP));
-    flux = e0 + 0.5 * x / dx * ((eP - e0) - (1. - 2. / 3. * x / dx) * six);
+    flux = e0 + k.p1 * ((eP - e0) - k.p2 * six);
   }
-  const double num = vdt / h[2], nup = vdt / h[3];
   const double i1lo = smin(ym, yp), i1hi = smax(ym, yp);
   double i2lo, i2hi;
-  if (vdt >= 0.) {
-    const double ext = ym + (1. - num) / num * (ym - y[1]);
+  if (k.pos) {
+    const double ext = ym + k.r * (ym - y[1]);
     i2lo = smin(ym, ext), i2hi = smax(ym, ext);
   } else {
-    const double ext = yp - (1. + nup) / nup * (yp - y[4]);
+    const double ext = yp - k.r * (yp - y[4]);
     i2lo = smin(yp, ext), i2hi = smax(yp, ext);
   }
   double lo, hi;
@@ -128,10 +163,7 @@ __device__ __forceinline__ double fluxOrdre3(const double* y, const double* h, d
 
 }  // namespace
 
-// O3: third-order face values (projectionOrder 3, no PenteBorne): cells
-// a-3 .. a+3 along the sweep, each neighbour lookup past the domain edge
-// returning the cell itself (remap/UtilesMesh.cc:5-51)
-template <int M, bool PP, bool XD, bool O3>
+template <int M, bool PP, bool XD>
 __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
   constexpr int NC = 3 * M + 1;
   const Opts& o = kp.o;
@@ -155,7 +187,7 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
   for (int d = 0; d < 5; ++d) {
     const long b = a + d - 2;
     in[d] = (b >= 0 && b < na);
-    if (O3 || !in[d]) continue;  // the third-order faces load their own stencil
+    if (!in[d]) continue;
     const long cd = c + (d - 2) * stride;
     double u[NC];
     const int nload = PP ? 3 * M : NC;
@@ -204,29 +236,7 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
   double phiM[NC], phiP[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) phiM[q] = phiP[q] = 0.0;
-  if constexpr (O3) {
-    // PhaseRemap1.cc:365-389, PhaseRemap2.cc:359-385: slot k holds the cell
-    // a-3+k, clamped to the line; the minus face uses slots 0..5, the plus
-    // face slots 1..6
-    long id[7];
-    double V[7], H[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-      const long b = min(max(a - 3 + k, 0L), na - 1);
-      id[k] = c + (b - a) * stride;
-      V[k] = f.VL[id[k]];
-      H[k] = widthC<XD>(kp, dt1, XD ? j : b, XD ? b : i);
-    }
-    const double vdtM = vm * dtn, vdtP = vp * dtn;
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-      double y[7];
-#pragma unroll
-      for (int k = 0; k < 7; ++k) y[k] = Uin[q][id[k]] / V[k];
-      if (hasM) phiM[q] = fluxOrdre3(y, H, vdtM);
-      if (hasP) phiP[q] = fluxOrdre3(y + 1, H + 1, vdtP);
-    }
-  } else if (!PP) {
+  if (!PP) {
     // upwind value (UtilesRemap-Impl.h:491-501) of a face between slots
     // lo and lo+1; back/front cells: x: left/right, y: above/below
     auto faceValue = [&](long b, const Slot<NC>& sl0, const Slot<NC>& sl1, const Slot<NC>& sl2, const Slot<NC>& sl3,
@@ -334,18 +344,157 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
   }
 }
 
-#define MHY_SWEEP_CELL_INST(M, PP, O3)                       \
-  template __global__ void k_sweep_cell<M, PP, true, O3>(KP); \
-  template __global__ void k_sweep_cell<M, PP, false, O3>(KP);
-MHY_SWEEP_CELL_INST(1, false, false)
-MHY_SWEEP_CELL_INST(1, true, false)
-MHY_SWEEP_CELL_INST(2, false, false)
-MHY_SWEEP_CELL_INST(2, true, false)
-MHY_SWEEP_CELL_INST(3, false, false)
-MHY_SWEEP_CELL_INST(3, true, false)
-MHY_SWEEP_CELL_INST(1, false, true)
-MHY_SWEEP_CELL_INST(2, false, true)
-MHY_SWEEP_CELL_INST(3, false, true)
+// Third-order sweep on an LDS tile: each thread loads one cell (its means
+// U/V_L and its width), the block then forms the faces of its inner cells
+// from LDS.  Along x a block is 256 cells of a row, 250 updated; along y
+// 8 columns x 32 rows, 26 rows updated.  Slots past the domain edge load the
+// edge cell, which is what the reference's neighbour lookups return
+// (remap/UtilesMesh.cc:5-51).  Arithmetic order follows the reference.
+namespace {
+constexpr int kO3Block = 256;
+constexpr int kO3XOut = kO3Block - 6;
+constexpr int kO3YCols = 8;
+constexpr int kO3YRows = kO3Block / kO3YCols;
+constexpr int kO3YOut = kO3YRows - 6;
+}  // namespace
+
+// width across the sweep faces from the end-of-Lagrange nodes, in either
+// storage form (HvLagrange, PhaseRemap1.cc:57-71, :104-119)
+template <bool XD, class R>
+__device__ __forceinline__ double widthO3(const KPT<R>& kp, double dt1, long j, long i) {
+  auto node = [&](long jj, long ii) {
+    const long p = jj * (kp.o.nx + 1) + ii;
+    const D2 e = eulerNode(kp.o, jj, ii);
+    return D2{e.x + (double)kp.f.ux[kp.nxt][p] * dt1, e.y + (double)kp.f.uy[kp.nxt][p] * dt1};
+  };
+  double h = 0.;
+  if (XD) {  // bottom then top face
+    h += 0.5 * d2norm(d2sub(node(j, i + 1), node(j, i)));
+    h += 0.5 * d2norm(d2sub(node(j + 1, i + 1), node(j + 1, i)));
+  } else {  // right then left face
+    h += 0.5 * d2norm(d2sub(node(j + 1, i + 1), node(j, i + 1)));
+    h += 0.5 * d2norm(d2sub(node(j + 1, i), node(j, i)));
+  }
+  return h;
+}
+
+// R: bulk storage type.  The face values are FP64 in both forms (the remap
+// vectors, V_L and the fluxes are FP64; the mixed form stores the node
+// velocities the widths come from in FP32).
+template <class R, int M, bool XD>
+__global__ __launch_bounds__(kO3Block) void k_sweep_o3(KPT<R> kp) {
+  constexpr int NC = 3 * M + 1;
+  __shared__ double sY[NC][kO3Block];
+  __shared__ double sH[kO3Block];
+  const Opts& o = kp.o;
+  const auto& f = kp.f;
+  const long nx = o.nx, ny = o.ny;
+  const int tid = threadIdx.x;
+  long a, t, na, nt;
+  int slot, last;
+  constexpr int S = XD ? 1 : kO3YCols;  // thread distance of the along-sweep neighbour
+  if (XD) {
+    const long nseg = (nx + kO3XOut - 1) / kO3XOut;
+    const long b = blockIdx.x;
+    t = b / nseg;
+    a = (b % nseg) * kO3XOut - 3 + tid;
+    slot = tid, last = kO3Block - 1;
+    na = nx, nt = ny;
+  } else {
+    const long gx = (nx + kO3YCols - 1) / kO3YCols;
+    const long b = blockIdx.x;
+    slot = tid / kO3YCols, last = kO3YRows - 1;
+    a = (b / gx) * kO3YOut - 3 + slot;
+    t = (b % gx) * kO3YCols + tid % kO3YCols;
+    na = ny, nt = nx;
+  }
+  const bool inT = (t < nt);
+  double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
+  const double dtn = f.sc->dt[kp.cur], dt1 = f.sc->dt[kp.nxt];
+  if (inT) {
+    const long b = min(max(a, 0L), na - 1);
+    const long j = XD ? t : b, i = XD ? b : t;
+    const long cb = j * nx + i;
+    const double V = f.VL[cb];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) sY[q][tid] = Uin[q][cb] / V;
+    sH[tid] = widthO3<XD>(kp, dt1, j, i);
+  }
+  __syncthreads();
+  if (!inT || a < 0 || a >= na || slot < 3 || slot > last - 3) return;
+
+  const long j = XD ? t : a, i = XD ? a : t;
+  const long c = j * nx + i;
+  // face normal velocity of face b, between along-cells b-1 and b
+  // (PrepareRemap.cc:50-62)
+  auto fnv = [&](long b) {
+    long n1, n2;
+    if (XD) n1 = j * (nx + 1) + b, n2 = (j + 1) * (nx + 1) + b;
+    else n1 = b * (nx + 1) + i, n2 = b * (nx + 1) + i + 1;
+    const R* u = XD ? f.ux[kp.nxt] : f.uy[kp.nxt];
+    return 0.0 + ((0.0 + (double)u[n1]) + (double)u[n2]) * 0.5;
+  };
+  // Euler length of face b (Init.cc:528-533)
+  auto faceLen = [&](long b) {
+    D2 e1, e2;
+    if (XD) e1 = eulerNode(o, j, b), e2 = eulerNode(o, j + 1, b);
+    else e1 = eulerNode(o, b, i), e2 = eulerNode(o, b, i + 1);
+    return d2norm(d2sub(e2, e1));
+  };
+  const bool hasM = (a >= 1), hasP = (a + 1 < na);
+  const double Lm = faceLen(a), Lp = faceLen(a + 1);
+  const double vm = fnv(a), vp = fnv(a + 1);
+  double H[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) H[k] = sH[tid + (k - 3) * S];
+  O3Grad gk[5];
+  O3Edge ek[4];
+#pragma unroll
+  for (int k = 1; k <= 5; ++k) gk[k - 1] = o3GradCoef(H[k - 1], H[k], H[k + 1]);
+#pragma unroll
+  for (int k = 1; k <= 4; ++k) ek[k - 1] = o3EdgeCoef(H[k - 1], H[k], H[k + 1], H[k + 2]);
+  const O3Sweep swM = o3SweepCoef(vm * dtn, H[2], H[3]), swP = o3SweepCoef(vp * dtn, H[3], H[4]);
+  double phiM[NC], phiP[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    double y[7], g[5], e[4];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) y[k] = sY[q][tid + (k - 3) * S];
+#pragma unroll
+    for (int k = 1; k <= 5; ++k) g[k - 1] = o3Grad(gk[k - 1], y[k - 1], y[k], y[k + 1]);
+#pragma unroll
+    for (int k = 1; k <= 4; ++k) e[k - 1] = o3Edge(ek[k - 1], y[k], y[k + 1], g[k - 1], g[k]);
+    phiM[q] = hasM ? o3Flux(swM, y, e[0], e[1], e[2]) : 0.0;
+    phiP[q] = hasP ? o3Flux(swP, y + 1, e[1], e[2], e[3]) : 0.0;
+  }
+  sweepUpdateCell<R, M, false, XD, double>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP);
+}
+
+dim3 sweepO3Grid(bool xd, long nx, long ny) {
+  if (xd) return dim3((unsigned)((nx + kO3XOut - 1) / kO3XOut * ny));
+  return dim3((unsigned)((nx + kO3YCols - 1) / kO3YCols * ((ny + kO3YOut - 1) / kO3YOut)));
+}
+
+#define MHY_SWEEP_O3_INST(R, M)                          \
+  template __global__ void k_sweep_o3<R, M, true>(KPT<R>); \
+  template __global__ void k_sweep_o3<R, M, false>(KPT<R>);
+MHY_SWEEP_O3_INST(double, 1)
+MHY_SWEEP_O3_INST(double, 2)
+MHY_SWEEP_O3_INST(double, 3)
+MHY_SWEEP_O3_INST(float, 1)
+MHY_SWEEP_O3_INST(float, 2)
+MHY_SWEEP_O3_INST(float, 3)
+#undef MHY_SWEEP_O3_INST
+
+#define MHY_SWEEP_CELL_INST(M, PP)                       \
+  template __global__ void k_sweep_cell<M, PP, true>(KP); \
+  template __global__ void k_sweep_cell<M, PP, false>(KP);
+MHY_SWEEP_CELL_INST(1, false)
+MHY_SWEEP_CELL_INST(1, true)
+MHY_SWEEP_CELL_INST(2, false)
+MHY_SWEEP_CELL_INST(2, true)
+MHY_SWEEP_CELL_INST(3, false)
+MHY_SWEEP_CELL_INST(3, true)
 #undef MHY_SWEEP_CELL_INST
 
 }  // namespace mhy

@@ -88,6 +88,9 @@ def extension_cases():
     c = _small(configs.triple_layout(), 48, 16)
     c.projection_order = 3
     out["C4_triple_layout_order3"] = c
+    c = configs.riemann(40, 0.43, 0.7, 0.6, 0.2, 0.1, 1.4, 1.67)
+    c.projection_order, c.methode_flux_masse = 3, 1  # A2: volume fluxes to the dual remap
+    out["riemann_order3_A2"] = c
     c = configs.riemann(32, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
     c.pseudo_centree = 1
     c.abort_on_energy = 1
@@ -98,6 +101,12 @@ def extension_cases():
     c.nx, c.ny, c.dx, c.dy = 40, 4, 0.025, 0.025
     c.limiter, c.pseudo_centree, c.abort_on_energy = abi.LIM["minmod"], 0, 0
     out["unit_test_boundary_flux"] = c
+    c = configs.default_config()
+    c.test_case, c.schema, c.nbmat = abi.CASE["UnitTestCase"], abi.SCHEMA_VNR, 1
+    c.nx, c.ny, c.dx, c.dy = 40, 4, 0.025, 0.025
+    c.limiter, c.pseudo_centree, c.abort_on_energy = abi.LIM["minmod"], 0, 0
+    c.projection_order = 3
+    out["unit_test_boundary_flux_order3"] = c
     return out
 
 
@@ -135,9 +144,6 @@ def rejected_cases():
     c = configs.riemann(16, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
     c.projection_order, c.pp = 3, 1
     out["order3_pentborne"] = c
-    c = configs.riemann(16, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
-    c.projection_order, c.precision = 3, 1
-    out["order3_mixed"] = c
     c = configs.riemann(16, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.4)
     c.projection_order = 4
     out["order4"] = c

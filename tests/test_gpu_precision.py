@@ -73,12 +73,7 @@ def test_mixed_triple_layout_long_run():
         mix.close()
 
 
-# the third-order remap runs in the FP64 form only (mhy_create rejects it in
-# the mixed form; test_gpu_parity checks that)
-MIXED_CASES = sorted(k for k, c in extension_cases().items() if c.projection_order != 3)
-
-
-@pytest.mark.parametrize("name", MIXED_CASES)
+@pytest.mark.parametrize("name", sorted(extension_cases()))
 def test_mixed_options(name):
     """Every option set of the parity suite (PenteBorne variants, A1/A2/PB
     dual mass fluxes, mixte flags, limiters, boundary fluxes) over 200 steps."""

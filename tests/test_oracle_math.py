@@ -366,6 +366,7 @@ RUNS = {
     "pp_PB": dict(limiter=LIM["superBee"], pp=1, methode_flux_masse=2),
     "order1": dict(projection_order=1),
     "order3": dict(projection_order=3),
+    "order3_A2": dict(projection_order=3, methode_flux_masse=1),
 }
 
 
