@@ -972,6 +972,14 @@ void (*sweepBatchKernel(bool xd, int m, bool pp))(BatchLaunchT<R>) {
   return t[xd ? 1 : 0][m - 1][pp ? 1 : 0];
 }
 template <class R, bool ONE>
+void (*o3BatchKernel(bool xd, int m))(BatchLaunchT<R>) {
+  using Fn = void (*)(BatchLaunchT<R>);
+  static const Fn t[2][3] = {
+      {k_sweep_o3_batch<R, 1, false, ONE>, k_sweep_o3_batch<R, 2, false, ONE>, k_sweep_o3_batch<R, 3, false, ONE>},
+      {k_sweep_o3_batch<R, 1, true, ONE>, k_sweep_o3_batch<R, 2, true, ONE>, k_sweep_o3_batch<R, 3, true, ONE>}};
+  return t[xd ? 1 : 0][m - 1];
+}
+template <class R, bool ONE>
 void (*walkBatchKernel(int m, bool pp))(BatchLaunchT<R>) {
   using Fn = void (*)(BatchLaunchT<R>);
   static const Fn t[3][2] = {{k_sweep_walk_batch<R, 1, false, ONE>, k_sweep_walk_batch<R, 1, true, ONE>},
@@ -989,6 +997,7 @@ struct mhy_batch {
   bool mixed = false;  // every problem in the mixed FP32 / FP64 form
   bool dualWaveY = true;  // nodal remap along y with the wave kernel
   bool walkY = false;     // cell sweeps along y with the column-walking kernel
+  bool o3 = false;        // third-order remap (LDS-tile cell sweeps)
   void* dTable = nullptr;  // KPT<double> or KPT<float> entries
   unsigned* dStart = nullptr;  // BK_COUNT x (n+1)
   unsigned* dGx = nullptr;     // BK_COUNT x n
@@ -1048,7 +1057,9 @@ int batchEnqueueStepR(mhy_batch* b, int cur, bool xty, bool timing) {
     const bool xd = (sweep == 1) ? xty : !xty;
     const int fuse = (sweep == 2) ? 1 : 0;
     const int v = cur + 2 * (sweep - 1) + 4 * (xd ? 1 : 0);
-    if (!xd && b->walkY)
+    if (b->o3)
+      launch(xd ? BK_SWX : BK_SWY, o3BatchKernel<R, false>(xd, b->m), o3BatchKernel<R, true>(xd, b->m), v + 8 * fuse);
+    else if (!xd && b->walkY)
       launch(BK_SWY, walkBatchKernel<R, false>(b->m, b->pp), walkBatchKernel<R, true>(b->m, b->pp), v + 8 * fuse);
     else
       launch(xd ? BK_SWX : BK_SWY, sweepBatchKernel<R, false>(xd, b->m, b->pp),
@@ -1108,15 +1119,16 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
     const mhy_config& c = cfgs[i];
     if (c.schema != MHY_SCHEMA_VNR || !c.avec_projection || c.sans_lagrange)
       return batchFail(b, MHY_ERR_ARG, "batch problems must use the VNR Lagrange + remap mode");
-    if (c.projection_order > 2)
-      return batchFail(b, MHY_ERR_ARG, "batch problems use a projection of order 1 or 2");
-    if (i > 0 && (c.nbmat != cfgs[0].nbmat || c.pp != cfgs[0].pp || c.precision != cfgs[0].precision))
+    if (i > 0 && (c.nbmat != cfgs[0].nbmat || c.pp != cfgs[0].pp || c.precision != cfgs[0].precision ||
+                  (c.projection_order == 3) != (cfgs[0].projection_order == 3)))
       return batchFail(b, MHY_ERR_ARG,
-                       "batch problems must share the material count, the PenteBorne option and the precision");
+                       "batch problems must share the material count, the PenteBorne option, the precision and "
+                       "whether the projection is of order 3");
   }
   b->m = cfgs[0].nbmat;
   b->pp = cfgs[0].pp == 1;
   b->mixed = cfgs[0].precision == 1;
+  b->o3 = cfgs[0].projection_order == 3;
   if (const char* dy = std::getenv("MAHYCO_DUAL_Y")) b->dualWaveY = (std::string(dy) != "node");
   if (const char* wy = std::getenv("MAHYCO_SWEEP_Y")) b->walkY = (std::string(wy) == "walk");
   if (const char* g = std::getenv("MAHYCO_BATCH_GRAPH")) b->useGraph = (std::string(g) != "0");
@@ -1163,8 +1175,12 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
           break;
         case BK_NODE: g = gridFor(x->nnode); break;
         case BK_CELL: g = gridFor(x->ncell); break;
-        case BK_SWX: g = sweepWaveGrid(true, nx, ny, b->mixed, b->pp); break;
-        case BK_SWY: g = b->walkY ? sweepWalkGrid(nx, ny) : sweepWaveGrid(false, nx, ny, b->mixed, b->pp); break;
+        case BK_SWX: g = b->o3 ? sweepO3Grid(true, nx, ny) : sweepWaveGrid(true, nx, ny, b->mixed, b->pp); break;
+        case BK_SWY:
+          g = b->o3      ? sweepO3Grid(false, nx, ny)
+              : b->walkY ? sweepWalkGrid(nx, ny)
+                         : sweepWaveGrid(false, nx, ny, b->mixed, b->pp);
+          break;
         case BK_DUX: g = dualWaveGrid(true, nx, ny, b->mixed); break;
         case BK_DUY: g = b->dualWaveY ? dualWaveGrid(false, nx, ny, b->mixed) : gridFor(x->nnode); break;
         default: g = gridFor(x->nnode); break;  // per-node dual along y

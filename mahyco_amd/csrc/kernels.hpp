@@ -37,6 +37,8 @@ __global__ void k_sweep_cell(KP kp);
 // third-order sweep on LDS tiles (256-thread blocks)
 template <class R, int M, bool XD>
 __global__ void k_sweep_o3(KPT<R> kp);
+template <class R, int M, bool XD, bool ONE>
+__global__ void k_sweep_o3_batch(BatchLaunchT<R> bl);
 dim3 sweepO3Grid(bool xd, long nx, long ny);
 template <class R, int M, bool PP, bool XD>
 __global__ void k_sweep_wave(KPT<R> kp);

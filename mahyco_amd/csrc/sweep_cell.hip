@@ -378,51 +378,17 @@ __device__ __forceinline__ double widthO3(const KPT<R>& kp, double dt1, long j, 
   return h;
 }
 
-// R: bulk storage type.  The face values are FP64 in both forms (the remap
-// vectors, V_L and the fluxes are FP64; the mixed form stores the node
-// velocities the widths come from in FP32).
+// the faces and the update of the inner cell at along index a, across
+// index t, whose tile slot is tid (sY: NC planes of kO3Block means)
 template <class R, int M, bool XD>
-__global__ __launch_bounds__(kO3Block) void k_sweep_o3(KPT<R> kp) {
+__device__ __forceinline__ void sweepO3Cell(const KPT<R>& kp, const double* sY, const double* sH, int tid, long a,
+                                            long t, long na) {
   constexpr int NC = 3 * M + 1;
-  __shared__ double sY[NC][kO3Block];
-  __shared__ double sH[kO3Block];
+  constexpr int S = XD ? 1 : kO3YCols;  // thread distance of the along-sweep neighbour
   const Opts& o = kp.o;
   const auto& f = kp.f;
-  const long nx = o.nx, ny = o.ny;
-  const int tid = threadIdx.x;
-  long a, t, na, nt;
-  int slot, last;
-  constexpr int S = XD ? 1 : kO3YCols;  // thread distance of the along-sweep neighbour
-  if (XD) {
-    const long nseg = (nx + kO3XOut - 1) / kO3XOut;
-    const long b = blockIdx.x;
-    t = b / nseg;
-    a = (b % nseg) * kO3XOut - 3 + tid;
-    slot = tid, last = kO3Block - 1;
-    na = nx, nt = ny;
-  } else {
-    const long gx = (nx + kO3YCols - 1) / kO3YCols;
-    const long b = blockIdx.x;
-    slot = tid / kO3YCols, last = kO3YRows - 1;
-    a = (b / gx) * kO3YOut - 3 + slot;
-    t = (b % gx) * kO3YCols + tid % kO3YCols;
-    na = ny, nt = nx;
-  }
-  const bool inT = (t < nt);
-  double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
-  const double dtn = f.sc->dt[kp.cur], dt1 = f.sc->dt[kp.nxt];
-  if (inT) {
-    const long b = min(max(a, 0L), na - 1);
-    const long j = XD ? t : b, i = XD ? b : t;
-    const long cb = j * nx + i;
-    const double V = f.VL[cb];
-#pragma unroll
-    for (int q = 0; q < NC; ++q) sY[q][tid] = Uin[q][cb] / V;
-    sH[tid] = widthO3<XD>(kp, dt1, j, i);
-  }
-  __syncthreads();
-  if (!inT || a < 0 || a >= na || slot < 3 || slot > last - 3) return;
-
+  const long nx = o.nx;
+  const double dtn = f.sc->dt[kp.cur];
   const long j = XD ? t : a, i = XD ? a : t;
   const long c = j * nx + i;
   // face normal velocity of face b, between along-cells b-1 and b
@@ -459,7 +425,7 @@ __global__ __launch_bounds__(kO3Block) void k_sweep_o3(KPT<R> kp) {
   for (int q = 0; q < NC; ++q) {
     double y[7], g[5], e[4];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) y[k] = sY[q][tid + (k - 3) * S];
+    for (int k = 0; k < 7; ++k) y[k] = sY[q * kO3Block + tid + (k - 3) * S];
 #pragma unroll
     for (int k = 1; k <= 5; ++k) g[k - 1] = o3Grad(gk[k - 1], y[k - 1], y[k], y[k + 1]);
 #pragma unroll
@@ -470,14 +436,77 @@ __global__ __launch_bounds__(kO3Block) void k_sweep_o3(KPT<R> kp) {
   sweepUpdateCell<R, M, false, XD, double>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP);
 }
 
+// One tile; blk: its linear block index.  R: bulk storage type.  The face
+// values are FP64 in both forms (the remap vectors, V_L and the fluxes are
+// FP64; the mixed form stores the node velocities the widths come from in
+// FP32).  Every thread of the block reaches both barriers (no early
+// return), so the ensemble kernels can run several tiles in one workgroup.
+template <class R, int M, bool XD>
+__device__ __forceinline__ void sweepO3Body(const KPT<R>& kp, unsigned blk) {
+  constexpr int NC = 3 * M + 1;
+  __shared__ double sY[NC][kO3Block];
+  __shared__ double sH[kO3Block];
+  const Opts& o = kp.o;
+  const auto& f = kp.f;
+  const long nx = o.nx, ny = o.ny;
+  const int tid = threadIdx.x;
+  long a, t, na, nt;
+  int slot, last;
+  if (XD) {
+    const long nseg = (nx + kO3XOut - 1) / kO3XOut;
+    const long b = blk;
+    t = b / nseg;
+    a = (b % nseg) * kO3XOut - 3 + tid;
+    slot = tid, last = kO3Block - 1;
+    na = nx, nt = ny;
+  } else {
+    const long gx = (nx + kO3YCols - 1) / kO3YCols;
+    const long b = blk;
+    slot = tid / kO3YCols, last = kO3YRows - 1;
+    a = (b / gx) * kO3YOut - 3 + slot;
+    t = (b % gx) * kO3YCols + tid % kO3YCols;
+    na = ny, nt = nx;
+  }
+  const bool inT = (t < nt);
+  double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
+  const double dt1 = f.sc->dt[kp.nxt];
+  if (inT) {
+    const long b = min(max(a, 0L), na - 1);
+    const long j = XD ? t : b, i = XD ? b : t;
+    const long cb = j * nx + i;
+    const double V = f.VL[cb];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) sY[q][tid] = Uin[q][cb] / V;
+    sH[tid] = widthO3<XD>(kp, dt1, j, i);
+  }
+  __syncthreads();
+  if (inT && a >= 0 && a < na && slot >= 3 && slot <= last - 3) sweepO3Cell<R, M, XD>(kp, sY[0], sH, tid, a, t, na);
+  __syncthreads();  // the ensemble kernels reuse the tile for their next block
+}
+
+template <class R, int M, bool XD>
+__global__ __launch_bounds__(kO3Block) void k_sweep_o3(KPT<R> kp) {
+  sweepO3Body<R, M, XD>(kp, blockIdx.x);
+}
+
+// all problems of an ensemble in one launch (BatchLaunchT, common.hpp)
+template <class R, int M, bool XD, bool ONE>
+__global__ __launch_bounds__(kO3Block) void k_sweep_o3_batch(BatchLaunchT<R> bl) {
+  batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) { sweepO3Body<R, M, XD>(kp, local); });
+}
+
 dim3 sweepO3Grid(bool xd, long nx, long ny) {
   if (xd) return dim3((unsigned)((nx + kO3XOut - 1) / kO3XOut * ny));
   return dim3((unsigned)((nx + kO3YCols - 1) / kO3YCols * ((ny + kO3YOut - 1) / kO3YOut)));
 }
 
-#define MHY_SWEEP_O3_INST(R, M)                          \
-  template __global__ void k_sweep_o3<R, M, true>(KPT<R>); \
-  template __global__ void k_sweep_o3<R, M, false>(KPT<R>);
+#define MHY_SWEEP_O3_INST(R, M)                                                \
+  template __global__ void k_sweep_o3<R, M, true>(KPT<R>);                       \
+  template __global__ void k_sweep_o3<R, M, false>(KPT<R>);                      \
+  template __global__ void k_sweep_o3_batch<R, M, true, false>(BatchLaunchT<R>);  \
+  template __global__ void k_sweep_o3_batch<R, M, false, false>(BatchLaunchT<R>); \
+  template __global__ void k_sweep_o3_batch<R, M, true, true>(BatchLaunchT<R>);   \
+  template __global__ void k_sweep_o3_batch<R, M, false, true>(BatchLaunchT<R>);
 MHY_SWEEP_O3_INST(double, 1)
 MHY_SWEEP_O3_INST(double, 2)
 MHY_SWEEP_O3_INST(double, 3)

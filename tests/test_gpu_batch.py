@@ -67,3 +67,31 @@ def test_batch_of_one_equals_the_problem():
     finally:
         single.close()
         batch.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_third_order_batch_matches_single_problems(precision):
+    """Ensembles at projection order 3 (LDS-tile sweeps) against the
+    problems alone.  About 6000 tiles per sweep, more than the 1024 to 8192
+    workgroups the tuning steps try, so those steps run the range-walking
+    kernels (several tiles per workgroup, the LDS tile reused) as well as
+    one workgroup per tile."""
+    cfgs = configs.ensemble(n=7, side_min=300, side_max=700, seed=77)
+    for c in cfgs:
+        c.projection_order = 3
+        c.precision = precision
+    batch = mahyco_amd.Batch(cfgs)
+    singles = [mahyco_amd.Context(c) for c in cfgs]
+    try:
+        log(f"[batch] order 3, {len(cfgs)} problems, {batch.cells} cells, precision={precision}")
+        done = 0
+        for chunk in (4, 6, 3):  # the tuning steps, then graph replays
+            batch.run(chunk)
+            for s in singles:
+                s.step(chunk)
+            done += chunk
+            _compare(batch, singles, cfgs, f"after {done} steps")
+    finally:
+        for s in singles:
+            s.close()
+        batch.close()
