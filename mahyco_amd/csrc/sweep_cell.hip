@@ -23,23 +23,23 @@ using namespace sw;
 
 namespace {
 
-// end-of-Lagrange node position (updatePosition, PhaseLagrange.cc:203-208)
-__device__ __forceinline__ D2 lagNodeC(const KP& kp, double dt1, long j, long i) {
-  const long p = j * (kp.o.nx + 1) + i;
-  const D2 e = eulerNode(kp.o, j, i);
-  return {e.x + kp.f.ux[kp.nxt][p] * dt1, e.y + kp.f.uy[kp.nxt][p] * dt1};
-}
-
-// width across the sweep faces (HvLagrange, PhaseRemap1.cc:57-71, :104-119)
-template <bool XD>
-__device__ __forceinline__ double widthC(const KP& kp, double dt1, long j, long i) {
+// width across the sweep faces (HvLagrange, PhaseRemap1.cc:57-71,
+// :104-119) from the end-of-Lagrange nodes (updatePosition,
+// PhaseLagrange.cc:203-208), in either storage form
+template <bool XD, class R>
+__device__ __forceinline__ double widthCell(const KPT<R>& kp, double dt1, long j, long i) {
+  auto node = [&](long jj, long ii) {
+    const long p = jj * (kp.o.nx + 1) + ii;
+    const D2 e = eulerNode(kp.o, jj, ii);
+    return D2{e.x + (double)kp.f.ux[kp.nxt][p] * dt1, e.y + (double)kp.f.uy[kp.nxt][p] * dt1};
+  };
   double h = 0.;
   if (XD) {  // bottom then top face
-    h += 0.5 * d2norm(d2sub(lagNodeC(kp, dt1, j, i + 1), lagNodeC(kp, dt1, j, i)));
-    h += 0.5 * d2norm(d2sub(lagNodeC(kp, dt1, j + 1, i + 1), lagNodeC(kp, dt1, j + 1, i)));
+    h += 0.5 * d2norm(d2sub(node(j, i + 1), node(j, i)));
+    h += 0.5 * d2norm(d2sub(node(j + 1, i + 1), node(j + 1, i)));
   } else {  // right then left face
-    h += 0.5 * d2norm(d2sub(lagNodeC(kp, dt1, j + 1, i + 1), lagNodeC(kp, dt1, j, i + 1)));
-    h += 0.5 * d2norm(d2sub(lagNodeC(kp, dt1, j + 1, i), lagNodeC(kp, dt1, j, i)));
+    h += 0.5 * d2norm(d2sub(node(j + 1, i + 1), node(j, i + 1)));
+    h += 0.5 * d2norm(d2sub(node(j + 1, i), node(j, i)));
   }
   return h;
 }
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_sweep_cell(KP kp) {
     for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cd] : 0.0;
     cellPhi<M, PP, double>(u, PP ? 1.0 : f.VL[cd], PP ? f.qL[cd] : 0.0, s[d].phi);
     s[d].xc = XD ? f.XcL[cd] : f.YcL[cd];
-    s[d].H = needH ? widthC<XD>(kp, dt1, XD ? j : b, XD ? b : i) : 0.0;
+    s[d].H = needH ? widthCell<XD>(kp, dt1, XD ? j : b, XD ? b : i) : 0.0;
     s[d].fl = 0;
     if (mixteOpt) s[d].fl = (kp.sweep == 1) ? packFlags(f.mixte[cd], f.pure[cd]) : packedFlags<M, double>(s[d].phi);
   }
@@ -358,26 +358,6 @@ constexpr int kO3YRows = kO3Block / kO3YCols;
 constexpr int kO3YOut = kO3YRows - 6;
 }  // namespace
 
-// width across the sweep faces from the end-of-Lagrange nodes, in either
-// storage form (HvLagrange, PhaseRemap1.cc:57-71, :104-119)
-template <bool XD, class R>
-__device__ __forceinline__ double widthO3(const KPT<R>& kp, double dt1, long j, long i) {
-  auto node = [&](long jj, long ii) {
-    const long p = jj * (kp.o.nx + 1) + ii;
-    const D2 e = eulerNode(kp.o, jj, ii);
-    return D2{e.x + (double)kp.f.ux[kp.nxt][p] * dt1, e.y + (double)kp.f.uy[kp.nxt][p] * dt1};
-  };
-  double h = 0.;
-  if (XD) {  // bottom then top face
-    h += 0.5 * d2norm(d2sub(node(j, i + 1), node(j, i)));
-    h += 0.5 * d2norm(d2sub(node(j + 1, i + 1), node(j + 1, i)));
-  } else {  // right then left face
-    h += 0.5 * d2norm(d2sub(node(j + 1, i + 1), node(j, i + 1)));
-    h += 0.5 * d2norm(d2sub(node(j + 1, i), node(j, i)));
-  }
-  return h;
-}
-
 // the faces and the update of the inner cell at along index a, across
 // index t, whose tile slot is tid (sY: NC planes of kO3Block means)
 template <class R, int M, bool XD>
@@ -477,7 +457,7 @@ __device__ __forceinline__ void sweepO3Body(const KPT<R>& kp, unsigned blk) {
     const double V = f.VL[cb];
 #pragma unroll
     for (int q = 0; q < NC; ++q) sY[q][tid] = Uin[q][cb] / V;
-    sH[tid] = widthO3<XD>(kp, dt1, j, i);
+    sH[tid] = widthCell<XD>(kp, dt1, j, i);
   }
   __syncthreads();
   if (inT && a >= 0 && a < na && slot >= 3 && slot <= last - 3) sweepO3Cell<R, M, XD>(kp, sY[0], sH, tid, a, t, na);
