@@ -58,6 +58,14 @@ def test_full_size_state_equals_oracle(name):
         orc.close()
 
 
+def _total_energy(ctx):
+    """Internal energy (cell mass x specific energy) plus kinetic energy
+    (node mass x |u|^2 / 2)."""
+    ein = float(np.sum(ctx.get("cell_mass") * ctx.get("energy")))
+    ux, uy, mp = ctx.get("velocity_x"), ctx.get("velocity_y"), ctx.get("node_mass")
+    return ein + float(np.sum(0.5 * mp * (ux * ux + uy * uy)))
+
+
 INVARIANTS = {
     # the workloads' own step counts
     "C2_riemann_2048_200_steps": (configs.riemann_perturbed, 200),
@@ -74,8 +82,10 @@ def test_full_size_run_conserves_mass_and_keeps_fractions(name):
     gpu = mahyco_amd.Context(cfg)
     try:
         m0 = np.array([gpu.get("mass_env", k).sum() for k in range(m)])
+        e0 = _total_energy(gpu)
         gpu.run(steps)
         m1 = np.array([gpu.get("mass_env", k).sum() for k in range(m)])
+        e1 = _total_energy(gpu)
         fr = np.array([gpu.get("fracvol", k) for k in range(m)])
         rho = gpu.get("density")
         for f in ("density", "pressure", "energy", "velocity_x", "velocity_y"):
@@ -83,7 +93,12 @@ def test_full_size_run_conserves_mass_and_keeps_fractions(name):
     finally:
         gpu.close()
     drift = np.abs(m1 - m0) / m0
-    log(f"[fullsize] {name}: mass drift {drift}")
+    log(f"[fullsize] {name}: mass drift {drift}, total energy {e0:.15e} -> {e1:.15e} "
+        f"(relative change {(e1 - e0) / e0:.3e})")
+    # the scheme does not conserve total energy to round-off (VNR's internal
+    # energy update and the nodal momentum remap): measured relative changes
+    # are -8.6e-9 (C2, 200 steps) and +2.1e-9 (C3, 100 steps)
+    assert np.isfinite(e1) and abs(e1 - e0) <= 1e-6 * e0
     assert np.all(drift <= 1e-11), drift
     assert fr.min() >= 0.0 and fr.max() <= 1.0
     np.testing.assert_allclose(fr.sum(axis=0), 1.0, rtol=0, atol=1e-12)
