@@ -54,6 +54,11 @@ struct mhy_ctx {
   // nodal remap along y: the wave kernel (default) or the per-node stencil
   // kernel (environment MAHYCO_DUAL_Y=node)
   bool dualWaveY = true;
+  // nodal remap along y with the column-walking kernel: chosen for the
+  // mixed form on meshes of at least 8192 walking waves, where it measured
+  // faster (fewer waves leave the chip underfilled); MAHYCO_DUAL_Y=walk /
+  // wave / node forces a kernel
+  bool dualWalkY = false;
   // cell sweeps along y: the column-walking kernel or the wave-strip
   // kernel.  The walker keeps more state in registers (fewer waves per SIMD)
   // but has less halo work; it is chosen for the mixed form with one
@@ -311,6 +316,10 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   } else if (xd) {
     timed(x, K_DUAL_X,
           [&] { launchR(x, k_dual<double, true>, k_dual<float, true>, gridFor(x->nnode), dim3(kBlock), kp); });
+  } else if (x->dualWalkY) {
+    timed(x, K_DUAL_Y, [&] {
+      launchR(x, k_dual_walk<double>, k_dual_walk<float>, dualWalkGrid(x->o.nx, x->o.ny), dim3(kBlock), kp);
+    });
   } else if (x->sweepWave && x->dualWaveY) {
     const dim3 grid = dualWaveGrid(false, x->o.nx, x->o.ny, x->mixed);
     timed(x, K_DUAL_Y,
@@ -452,7 +461,6 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   x->cfg = *cfg;
   if (const char* sk = std::getenv("MAHYCO_SWEEP_KERNEL")) x->sweepWave = (std::string(sk) != "cell");
   if (const char* ff = std::getenv("MAHYCO_FUSE_FINAL")) x->fuseFinal = (std::string(ff) != "0");
-  if (const char* dy = std::getenv("MAHYCO_DUAL_Y")) x->dualWaveY = (std::string(dy) != "node");
   const mhy_config& c = *cfg;
   auto bail = [&](int code) {
     // keep the context so the caller can read the message, unless it is
@@ -470,6 +478,11 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   x->mixed = (c.precision == 1);
   if (x->mixed) x->sweepWave = true;  // the per-cell stencil sweeps are FP64 only
   x->walkY = x->mixed && c.nbmat == 1 && c.pp != 1 && (long)sweepWalkGrid(c.nx, c.ny).x * 4 >= 4096;
+  x->dualWalkY = x->mixed && (long)dualWalkGrid(c.nx, c.ny).x * 4 >= 8192;
+  if (const char* dy = std::getenv("MAHYCO_DUAL_Y")) {
+    x->dualWaveY = (std::string(dy) != "node");
+    x->dualWalkY = (std::string(dy) == "walk");
+  }
   if (const char* wy = std::getenv("MAHYCO_SWEEP_Y")) {
     if (std::string(wy) == "walk") x->walkY = true;
     if (std::string(wy) == "wave") x->walkY = false;

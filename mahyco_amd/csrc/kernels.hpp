@@ -55,6 +55,10 @@ __global__ void k_dual(KPT<R> kp);
 template <class R, bool XD>
 __global__ void k_dual_wave(KPT<R> kp);
 dim3 dualWaveGrid(bool xd, long nx, long ny, bool mixed);
+// nodal remap along y with one lane per node column walking down a band
+template <class R>
+__global__ void k_dual_walk(KPT<R> kp);
+dim3 dualWalkGrid(long nx, long ny);
 unsigned dualWaveBlock(bool xd, bool mixed);
 template <class R, int M>
 __global__ void k_remap_final(KPT<R> kp);

@@ -212,6 +212,102 @@ __device__ __forceinline__ double dshfl(double v, int src) { return __shfl(v, sr
 __device__ __forceinline__ float dshfl(float v, int src) { return __shfl(v, src, kDW); }
 }  // namespace
 
+// The nodal update of node p (along index a, across index t) from its
+// DualPhi and Lagrange coordinate and those of its two along-sweep
+// neighbours (dpx, dpy, xl: nodes a-1, a, a+1), its limited gradient g0 and
+// the neighbours' gM, gP (zero outside the domain): mass fluxes through the
+// dual faces, upwind velocities, and the nodal vector of sweep 1 or the
+// velocity of sweep 2.
+template <class R, bool XD, class T>
+__device__ __forceinline__ void dualNodeFinish(const KPT<R>& kp, long a, long t, long na, long nt, long p,
+                                               const T* dpx, const T* dpy, const double* xl, const T* g0,
+                                               const T* gM, const T* gP) {
+  const Opts& o = kp.o;
+  const auto& f = kp.f;
+  const int m = o.nbmat;
+  const long nx = o.nx;
+  const double dtn = f.sc->dt[kp.cur];
+  const int order2 = o.order - 1;
+
+  // nodal mass fluxes through the dual faces on the minus / plus side (FP64)
+  double plus = 0., minus = 0.;
+  {
+    const bool interior = (a > 0 && a < na);
+    long tf[2];
+    int nf = 0;
+    if (interior) {
+      if (t >= 1) tf[nf++] = t - 1;
+      if (t <= nt - 1) tf[nf++] = t;
+    }
+    auto gidx = [&](long b, long tt) { return XD ? tt * (nx + 1) + b : b * nx + tt; };
+    auto cidx = [&](long b, long tt) { return XD ? tt * nx + b : b * nx + tt; };
+    for (int k = 0; k < m; ++k) {
+      double pk = 0, mk = 0;
+      if (o.methode == 0 || o.methode == 1) {
+        for (int s2 = 0; s2 < nf; ++s2) {
+          const long tt = tf[s2];
+          const double Gf = f.G[k][gidx(a, tt)];
+          double tp = Gf + f.G[k][gidx(a + 1, tt)];
+          double tm = Gf + f.G[k][gidx(a - 1, tt)];
+          if (o.methode == 1) tp = f.rL[cidx(a, tt)] * tp, tm = f.rL[cidx(a - 1, tt)] * tm;
+          pk += tp;
+          mk += tm;
+        }
+        const int nbfaces = 2 * nf;
+        if (nbfaces != 0) pk /= nbfaces, mk /= nbfaces;
+      } else if (o.methode == 2) {
+        int nbcell = 0;
+        for (int s2 = 0; s2 < nf; ++s2) {
+          const long tt = tf[s2];
+          pk += f.DF[k][cidx(a, tt)];
+          mk += f.DF[k][cidx(a - 1, tt)];
+          nbcell++;
+        }
+        if (nbcell != 0) pk /= nbcell, mk /= nbcell;
+      }
+      plus += pk;
+      minus += mk;
+    }
+  }
+  // upwind velocities (UtilesDualRemap.cc:234-397); a missing neighbour
+  // reads as 0 and its mass flux is 0
+  const T dtnT = T(dtn);
+  auto upwind = [&](bool minusSide, double flux, const T* gNb, double* r) {
+    const int dn = minusSide ? 0 : 2;
+    const T dNb[2] = {dpx[dn], dpy[dn]}, d0[2] = {dpx[1], dpy[1]};
+    const T uN = XD ? dpx[dn] : dpy[dn], u0 = XD ? dpx[1] : dpy[1];
+    const T u = minusSide ? T(0.5) * (uN + u0) : T(0.5) * (u0 + uN);
+    const int sign = (u > T(0)) ? 1 : -1;
+    const T h = T(minusSide ? (xl[1] - xl[dn]) : (xl[dn] - xl[1]));
+    const bool fromNb = minusSide ? (flux > 0) : (flux < 0);
+    for (int d = 0; d < 2; ++d) {
+      const T base = fromNb ? dNb[d] : d0[d];
+      const T g = fromNb ? gNb[d] : g0[d];
+      r[d] = (double)(base + T(order2) * (T(0.5) * g * (T(sign) * h - dtnT * u)));
+    }
+  };
+  double uvm[2], uvp[2];
+  upwind(true, minus, gM, uvm);
+  upwind(false, plus, gP, uvp);
+  double U0, U1, U2;
+  if (kp.sweep == 1) {
+    const double Mp = f.Mp[p];
+    U0 = Mp * f.ux[kp.nxt][p], U1 = Mp * f.uy[kp.nxt][p], U2 = Mp;
+  } else {
+    U0 = f.UD1[0][p], U1 = f.UD1[1][p], U2 = f.UD1[2][p];
+  }
+  const double R2 = U2 + minus - plus;
+  const double R0 = U0 + minus * uvm[0] - plus * uvp[0];
+  const double R1 = U1 + minus * uvm[1] - plus * uvp[1];
+  if (kp.sweep == 1) {
+    f.UD1[0][p] = R0, f.UD1[1][p] = R1, f.UD1[2][p] = R2;
+  } else {
+    // remapVariables, nodes (PhaseRemapFinal.cc:207-234)
+    f.ux[kp.nxt][p] = R0 / R2;
+    f.uy[kp.nxt][p] = R1 / R2;
+  }
+}
+
 // b: linear block index (strips numbered row-major along x, as the sweeps)
 template <class R, bool XD>
 __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
@@ -221,7 +317,6 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   constexpr int S = XD ? 1 : kDYCols;  // lane distance of the along-sweep neighbour
   const Opts& o = kp.o;
   const auto& f = kp.f;
-  const int m = o.nbmat;
   const long nx = o.nx, ny = o.ny;
   const int lane = threadIdx.x & (kDW - 1), wave = threadIdx.x >> 6;
   long a, t, na, nt;
@@ -246,9 +341,7 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   const long j = XD ? t : a, i = XD ? a : t;
   const long p = j * (nx + 1) + i;
   const bool output = in && slot >= 3 && slot <= last - 3;
-  const double dtn = f.sc->dt[kp.cur];
   const double dt1 = f.sc->dt[kp.nxt];
-  const int order2 = o.order - 1;
 
   // own DualPhi and Lagrange coordinate (coordinates stay FP64: only their
   // differences enter T arithmetic)
@@ -321,84 +414,115 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   // a neighbour outside the domain has a zero gradient
   if (!inS[2]) gM[0] = gM[1] = T(0.0);
   if (!inS[4]) gP[0] = gP[1] = T(0.0);
+  dualNodeFinish<R, XD, T>(kp, a, t, na, nt, p, &dpx[2], &dpy[2], &xl[2], g0, gM, gP);
+}
 
-  // nodal mass fluxes through the dual faces on the minus / plus side (FP64)
-  double plus = 0., minus = 0.;
-  {
-    const bool interior = (a > 0 && a < na);
-    long tf[2];
-    int nf = 0;
-    if (interior) {
-      if (t >= 1) tf[nf++] = t - 1;
-      if (t <= nt - 1) tf[nf++] = t;
-    }
-    auto gidx = [&](long b, long tt) { return XD ? tt * (nx + 1) + b : b * nx + tt; };
-    auto cidx = [&](long b, long tt) { return XD ? tt * nx + b : b * nx + tt; };
-    for (int k = 0; k < m; ++k) {
-      double pk = 0, mk = 0;
-      if (o.methode == 0 || o.methode == 1) {
-        for (int s2 = 0; s2 < nf; ++s2) {
-          const long tt = tf[s2];
-          const double Gf = f.G[k][gidx(a, tt)];
-          double tp = Gf + f.G[k][gidx(a + 1, tt)];
-          double tm = Gf + f.G[k][gidx(a - 1, tt)];
-          if (o.methode == 1) tp = f.rL[cidx(a, tt)] * tp, tm = f.rL[cidx(a - 1, tt)] * tm;
-          pk += tp;
-          mk += tm;
-        }
-        const int nbfaces = 2 * nf;
-        if (nbfaces != 0) pk /= nbfaces, mk /= nbfaces;
-      } else if (o.methode == 2) {
-        int nbcell = 0;
-        for (int s2 = 0; s2 < nf; ++s2) {
-          const long tt = tf[s2];
-          pk += f.DF[k][cidx(a, tt)];
-          mk += f.DF[k][cidx(a - 1, tt)];
-          nbcell++;
-        }
-        if (nbcell != 0) pk /= nbcell, mk /= nbcell;
+// Nodal remap along y with one lane per node column: a wave covers 64
+// consecutive node columns and walks down a band of kDWalkRows node rows,
+// plus 3 halo rows above and below, so every row access is a whole 64-node
+// row segment.  Each iteration loads one new row r into a 7-row register
+// window, forms the divided difference between rows r-1 and r, the limited
+// gradient of row r-2 and the update of row r-3.  Every value is formed
+// from the same operands as in the wave-strip kernel, so results are
+// bitwise the same.
+namespace {
+constexpr int kDWalkRows = 32;
+}  // namespace
+
+template <class R>
+__device__ __forceinline__ void dualWalkBody(const KPT<R>& kp, long item) {
+  using T = typename std::conditional<kMixed<R>, float, double>::type;
+  const Opts& o = kp.o;
+  const auto& f = kp.f;
+  const long nx = o.nx, na = o.ny, nt = o.nx;
+  const long gx = (nx + 1 + kDW - 1) / kDW;
+  const long t = (item % gx) * kDW + (threadIdx.x & (kDW - 1));
+  const long a0 = (item / gx) * kDWalkRows;                               // first row updated
+  const long a1 = (a0 + kDWalkRows - 1 < na) ? a0 + kDWalkRows - 1 : na;  // last row updated
+  const bool inT = (t <= nt);
+  const double dt1 = f.sc->dt[kp.nxt];
+  const bool ord = o.order > 1;
+  // window slot k holds row r-6+k; gradients of rows r-4, r-3, r-2; divided
+  // differences of rows r-2, r-1, r (towards the row above)
+  T wx[7], wy[7], g0x[3], g0y[3], glx[3], gly[3];
+  double wl[7];
+  bool wa[7];  // row inside [0, na]
+#pragma unroll
+  for (int k = 0; k < 7; ++k) wx[k] = wy[k] = T(0.0), wl[k] = 0.0, wa[k] = false;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g0x[k] = g0y[k] = glx[k] = gly[k] = T(0.0);
+  for (long r = a0 - 3; r <= a1 + 3; ++r) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) wx[k] = wx[k + 1], wy[k] = wy[k + 1], wl[k] = wl[k + 1], wa[k] = wa[k + 1];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) g0x[k] = g0x[k + 1], g0y[k] = g0y[k + 1], glx[k] = glx[k + 1], gly[k] = gly[k + 1];
+    // own DualPhi and Lagrange coordinate of row r
+    const bool rin = (r >= 0 && r <= na);
+    T dx = T(0.0), dy = T(0.0);
+    double xl = 0.0;
+    if (rin && inT) {
+      const long p = r * (nx + 1) + t;
+      if (kp.sweep == 1) {
+        const double ux1 = (double)f.ux[kp.nxt][p], uy1 = (double)f.uy[kp.nxt][p];
+        dx = T(ux1), dy = T(uy1);
+        const D2 e = eulerNode(o, r, t);
+        xl = e.y + uy1 * dt1;
+        if (r >= a0 && r <= a1) f.XL2[p] = e.x + ux1 * dt1;  // the second sweep's axis
+      } else {
+        const double M1 = f.UD1[2][p];
+        dx = T(f.UD1[0][p] / M1), dy = T(f.UD1[1][p] / M1);
+        xl = f.XL2[p];
       }
-      plus += pk;
-      minus += mk;
+    }
+    wx[6] = dx, wy[6] = dy, wl[6] = xl, wa[6] = rin;
+    // divided difference of row r towards row r-1 (zeros outside the domain)
+    glx[2] = gly[2] = T(0.0);
+    if (ord) {
+      glx[2] = (wx[6] - wx[5]) / T(wl[6] - wl[5]);
+      gly[2] = (wy[6] - wy[5]) / T(wl[6] - wl[5]);
+    }
+    // limited gradient of row c = r-2 (window slot 4)
+    g0x[2] = g0y[2] = T(0.0);
+    if (wa[4] && inT && ord && wa[3] && wa[5]) {
+      const T h0 = T(0.5 * (wl[5] - wl[3]));
+      T hm, hp;
+      if (!wa[2]) {
+        hm = T(0.), hp = T(0.5 * (wl[6] - wl[4]));
+      } else if (!wa[6]) {
+        hp = T(0.), hm = T(0.5 * (wl[4] - wl[2]));
+      } else {
+        hm = T(0.5 * (wl[4] - wl[2]));
+        hp = T(0.5 * (wl[6] - wl[4]));
+      }
+      g0x[2] = sw::limitedGradient<T>(o.limiter, glx[1], glx[0], wx[5], wx[3], h0, hp, hm);
+      g0y[2] = sw::limitedGradient<T>(o.limiter, gly[1], gly[0], wy[5], wy[3], h0, hp, hm);
+    }
+    // update of row a = r-3 (window slot 3)
+    const long a = r - 3;
+    if (a >= a0 && a <= a1 && wa[3] && inT) {
+      const T dpx[3] = {wx[2], wx[3], wx[4]}, dpy[3] = {wy[2], wy[3], wy[4]};
+      const double xls[3] = {wl[2], wl[3], wl[4]};
+      const T g0[2] = {g0x[1], g0y[1]};
+      T gM[2] = {g0x[0], g0y[0]}, gP[2] = {g0x[2], g0y[2]};
+      if (!wa[2]) gM[0] = gM[1] = T(0.0);
+      if (!wa[4]) gP[0] = gP[1] = T(0.0);
+      dualNodeFinish<R, false, T>(kp, a, t, na, nt, a * (nx + 1) + t, dpx, dpy, xls, g0, gM, gP);
     }
   }
-  // upwind velocities (UtilesDualRemap.cc:234-397); a missing neighbour
-  // reads as 0 and its mass flux is 0
-  const T dtnT = T(dtn);
-  auto upwind = [&](bool minusSide, double flux, const T* gNb, double* r) {
-    const int dn = minusSide ? 2 : 4;
-    const T dNb[2] = {dpx[dn], dpy[dn]}, d0[2] = {dpx[3], dpy[3]};
-    const T uN = XD ? dpx[dn] : dpy[dn], u0 = XD ? dpx[3] : dpy[3];
-    const T u = minusSide ? T(0.5) * (uN + u0) : T(0.5) * (u0 + uN);
-    const int sign = (u > T(0)) ? 1 : -1;
-    const T h = T(minusSide ? (xl[3] - xl[dn]) : (xl[dn] - xl[3]));
-    const bool fromNb = minusSide ? (flux > 0) : (flux < 0);
-    for (int d = 0; d < 2; ++d) {
-      const T base = fromNb ? dNb[d] : d0[d];
-      const T g = fromNb ? gNb[d] : g0[d];
-      r[d] = (double)(base + T(order2) * (T(0.5) * g * (T(sign) * h - dtnT * u)));
-    }
-  };
-  double uvm[2], uvp[2];
-  upwind(true, minus, gM, uvm);
-  upwind(false, plus, gP, uvp);
-  double U0, U1, U2;
-  if (kp.sweep == 1) {
-    const double Mp = f.Mp[p];
-    U0 = Mp * f.ux[kp.nxt][p], U1 = Mp * f.uy[kp.nxt][p], U2 = Mp;
-  } else {
-    U0 = f.UD1[0][p], U1 = f.UD1[1][p], U2 = f.UD1[2][p];
-  }
-  const double R2 = U2 + minus - plus;
-  const double R0 = U0 + minus * uvm[0] - plus * uvp[0];
-  const double R1 = U1 + minus * uvm[1] - plus * uvp[1];
-  if (kp.sweep == 1) {
-    f.UD1[0][p] = R0, f.UD1[1][p] = R1, f.UD1[2][p] = R2;
-  } else {
-    // remapVariables, nodes (PhaseRemapFinal.cc:207-234)
-    f.ux[kp.nxt][p] = R0 / R2;
-    f.uy[kp.nxt][p] = R1 / R2;
-  }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_dual_walk(KPT<R> kp) {
+  const long gx = (kp.o.nx + 1 + kDW - 1) / kDW;
+  const long bands = (kp.o.ny + 1 + kDWalkRows - 1) / kDWalkRows;
+  const long item = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (item >= gx * bands) return;  // the whole wave
+  dualWalkBody<R>(kp, item);
+}
+
+dim3 dualWalkGrid(long nx, long ny) {
+  const long gx = (nx + 1 + kDW - 1) / kDW, bands = (ny + 1 + kDWalkRows - 1) / kDWalkRows;
+  return dim3((unsigned)((gx * bands + 3) / 4));
 }
 
 template <class R, bool XD>
@@ -509,6 +633,7 @@ __global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
   template __global__ void k_dual_batch<R, false, true>(BatchLaunchT<R>);  \
   template __global__ void k_dual_wave<R, true>(KPT<R>);               \
   template __global__ void k_dual_wave<R, false>(KPT<R>);              \
+  template __global__ void k_dual_walk<R>(KPT<R>);                     \
   template __global__ void k_dual_wave_batch<R, true, false>(BatchLaunchT<R>);  \
   template __global__ void k_dual_wave_batch<R, false, false>(BatchLaunchT<R>); \
   template __global__ void k_dual_wave_batch<R, true, true>(BatchLaunchT<R>);   \

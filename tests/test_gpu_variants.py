@@ -1,8 +1,10 @@
 """The kernel variants that the environment selects must give bitwise the
-state of the default kernels (FP64 form): the per-cell stencil sweeps
-(MAHYCO_SWEEP_KERNEL=cell), the separate final-remap kernel
-(MAHYCO_FUSE_FINAL=0) and the per-node dual remap along y
-(MAHYCO_DUAL_Y=node).  The choice is read when a context is created."""
+state of the default kernels: the per-cell stencil sweeps
+(MAHYCO_SWEEP_KERNEL=cell, FP64 form only), the separate final-remap kernel
+(MAHYCO_FUSE_FINAL=0), the per-node dual remap along y (MAHYCO_DUAL_Y=node,
+FP64 form: in the mixed form it keeps FP64 velocity arithmetic) and the
+column-walking dual remap along y (MAHYCO_DUAL_Y=walk), in both storage
+forms.  The choice is read when a context is created."""
 import os
 from contextlib import contextmanager
 
@@ -17,7 +19,8 @@ from test_gpu_parity import extension_cases
 pytestmark = pytest.mark.gpu
 STEPS = 12
 VARIANTS = {"cell_sweeps": ("MAHYCO_SWEEP_KERNEL", "cell"), "separate_final": ("MAHYCO_FUSE_FINAL", "0"),
-            "node_dual_y": ("MAHYCO_DUAL_Y", "node")}
+            "node_dual_y": ("MAHYCO_DUAL_Y", "node"), "walk_dual_y": ("MAHYCO_DUAL_Y", "walk"),
+            "wave_dual_y": ("MAHYCO_DUAL_Y", "wave")}
 NAMES = ["C2_pp_superbee", "C2_pp_PB", "C2_pp_mixte_arithmeticG", "C3_point_explosion", "C4_triple_layout",
          "riemann_order1", "riemann_superBeeG_main", "unit_test_boundary_flux", "riemann_order3"]
 
@@ -35,12 +38,19 @@ def env(name, value):
             os.environ[name] = old
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
 @pytest.mark.parametrize("name", NAMES)
-def test_variant_equals_default(name, variant):
+def test_variant_equals_default(name, variant, precision):
+    if precision == 1 and variant == "cell_sweeps":
+        pytest.skip("the per-cell stencil sweeps are FP64 only (the mixed form ignores the switch)")
+    if precision == 1 and variant == "node_dual_y":
+        pytest.skip("the per-node dual kernel keeps FP64 velocity arithmetic in the mixed form, "
+                    "the wave and walking kernels FP32: equal only to tolerance")
     cfg = extension_cases()[name]
+    cfg.precision = precision
     var, value = VARIANTS[variant]
-    log(f"[variant] {variant} {name} {cfg.nx}x{cfg.ny} m={cfg.nbmat}")
+    log(f"[variant] {variant} {name} {cfg.nx}x{cfg.ny} m={cfg.nbmat} precision={precision}")
     ref = mahyco_amd.Context(cfg)
     with env(var, value):
         alt = mahyco_amd.Context(cfg)
