@@ -9,7 +9,9 @@ Default workload: the two-material Riemann problem on 2048^2 with a
 perturbed interface, superBee + PenteBorne, FP64.  --config c4 runs the
 three-material layout in the mixed FP32 / FP64 form with a per-field error
 report against the FP64 form after the run; --config c5 runs the 512-problem
-ensemble batched (mixed form by default) against the plain FP64 form.
+ensemble batched (mixed form by default) against the plain FP64 form;
+--config c1 runs the reference's own test input (Sod X on 200x5), whose
+1000 cells leave the step bound by launch latency.
 
 Prints one JSON line.  --gpus N > 1 exits: the library runs on one GPU.
 """
@@ -32,6 +34,10 @@ DTYPE = {0: "f64", 1: "f32+f64"}
 
 def workload(name):
     from mahyco_amd import configs
+    if name == "c1":
+        import mahyco_amd
+        path = os.path.join(ROOT, "tests", "golden", "vnr", "SOD_X_Mono_mat_Minmod_vnr_o2", "donnees.txt")
+        return "Sod X 200x5, 1 material, minmod order 2 (the reference's own test input)", mahyco_amd.read_donnees(path)
     if name == "c2":
         return "two-material Riemann 2048x2048, perturbed interface, superBee+PenteBorne", configs.riemann_perturbed()
     if name == "c3":
@@ -285,6 +291,17 @@ def main():
     }
     if prec == 1 and args.report_steps > 0:
         out["precision_report"] = precision_report(cfg, args.report_steps)
+    if args.config == "c1":
+        # a mesh this small is bound by launch latency: the same problem as a
+        # batch of one replays each step as a captured HIP graph
+        batch = mahyco_amd.Batch([cfg])
+        batch.run(max(args.warmup, 12))  # the first 10 steps tune the launch widths
+        t0 = time.perf_counter()
+        batch.run(args.steps)
+        gwall = time.perf_counter() - t0
+        batch.close()
+        out["graph_form"] = {"value": cells * args.steps / gwall, "ms_per_step": gwall / args.steps * 1e3,
+                             "note": "the same problem as a batch of one (captured HIP graphs)"}
     if not args.no_cpu_baseline:
         cfg.precision = 0
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
