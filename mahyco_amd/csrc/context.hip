@@ -65,6 +65,11 @@ struct mhy_ctx {
   // material and limited slopes on meshes of at least 4096 walking waves,
   // where it measured faster.  MAHYCO_SWEEP_Y=walk / wave forces either.
   bool walkY = false;
+  // mhy_run replays each step as a captured HIP graph, one per step parity
+  // and sweep order (index 2 * cur + x_then_y_n); MAHYCO_GRAPH=0 launches
+  // eagerly
+  bool useGraph = true;
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
   int status = MHY_OK;
   long ncell = 0, nnode = 0, nface = 0;
   std::string err;
@@ -487,6 +492,7 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     if (std::string(wy) == "walk") x->walkY = true;
     if (std::string(wy) == "wave") x->walkY = false;
   }
+  if (const char* g = std::getenv("MAHYCO_GRAPH")) x->useGraph = (std::string(g) != "0");
   if (c.sans_lagrange && !c.avec_projection)
     return bail(fail(x, MHY_ERR_ARG, "sansLagrange needs the projection"));
   std::string err;
@@ -634,6 +640,8 @@ int mhy_destroy(mhy_ctx* x) {
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   for (auto& r : x->pending) (void)hipEventDestroy(r.a), (void)hipEventDestroy(r.b);
   for (auto ev : x->eventPool) (void)hipEventDestroy(ev);
+  for (auto& g : x->graph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (void* p : x->allocs) (void)hipFree(p);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x->writer;
@@ -655,12 +663,47 @@ int mhy_step(mhy_ctx* x, int sync) {
   return MHY_OK;
 }
 
+// Captures the step that starts with the current parity and sweep order.
+// Capturing records the launches without running them, so the host-side
+// step state (iteration count, fused-final flag) is put back afterwards.
+int captureStep(mhy_ctx* x) {
+  const long it = x->iteration;
+  hipGraph_t g = nullptr;
+  HIPCHK(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  const int st = runIteration(x);
+  const hipError_t e = hipStreamEndCapture(x->stream, &g);
+  x->iteration = it;
+  x->finalDone = false;
+  if (st != MHY_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return fail(x, MHY_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&x->graph[2 * x->cur + (x->x_then_y_n ? 1 : 0)], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(x, MHY_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+  return MHY_OK;
+}
+
 int mhy_run(mhy_ctx* x, int nsteps) {
   if (!x) return MHY_ERR_ARG;
   if (x->status != MHY_OK) return x->status;
+  // graphs need the launches of a step to depend only on the parity; kernel
+  // events (timing) are recorded on the eager path
+  const bool graphs = x->useGraph && !x->timing && nsteps >= 2;
   for (int s = 0; s < nsteps; ++s) {
-    int st = runIteration(x);
-    if (st != MHY_OK) return st;
+    if (graphs) {
+      hipGraphExec_t& g = x->graph[2 * x->cur + (x->x_then_y_n ? 1 : 0)];
+      if (!g) {
+        const int st = captureStep(x);
+        if (st != MHY_OK) return st;
+      }
+      HIPCHK(x, hipGraphLaunch(g, x->stream));
+      x->iteration++;
+    } else {
+      int st = runIteration(x);
+      if (st != MHY_OK) return st;
+    }
     endIteration(x);
   }
   Scalars s;
