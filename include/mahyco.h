@@ -259,11 +259,16 @@ int mhy_batch_context(mhy_batch* batch, int i, mhy_ctx** ctx);
 long long mhy_batch_cells(const mhy_batch* batch);
 
 /* Timing of the kernels launched by the last mhy_run (HIP events on the
- * context stream): per-kernel average milliseconds, by kernel index. */
+ * context stream): per-kernel average milliseconds, by kernel index.
+ * mhy_set_kernel_timing_mask limits the events to the kernels whose bit
+ * (1 << k) is set (default: all), so that timing one kernel perturbs the
+ * step as little as possible.  No counterpart in the reference, which has
+ * no per-kernel timers. */
 int mhy_kernel_count(const mhy_ctx* ctx);
 const char* mhy_kernel_name(const mhy_ctx* ctx, int k);
 int mhy_kernel_bytes(const mhy_ctx* ctx, int k, double* bytes_per_launch);
 int mhy_enable_kernel_timing(mhy_ctx* ctx, int on);
+int mhy_set_kernel_timing_mask(mhy_ctx* ctx, unsigned mask);
 int mhy_kernel_time(mhy_ctx* ctx, int k, double* avg_ms, int* launches);
 
 #ifdef __cplusplus

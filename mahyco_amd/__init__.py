@@ -21,7 +21,7 @@ EXPORTS = [
     "mhy_prepare_remap", "mhy_remap_sweep", "mhy_remap_stage",
     "mhy_remap_final", "mhy_end_iteration", "mhy_synchronize", "mhy_get_time_state", "mhy_get_mesh_size",
     "mhy_get_field", "mhy_set_field", "mhy_conservation", "mhy_kernel_count", "mhy_kernel_name",
-    "mhy_kernel_bytes", "mhy_enable_kernel_timing", "mhy_kernel_time",
+    "mhy_kernel_bytes", "mhy_enable_kernel_timing", "mhy_set_kernel_timing_mask", "mhy_kernel_time",
     "mhy_batch_create", "mhy_batch_destroy", "mhy_batch_last_error", "mhy_batch_run", "mhy_batch_context",
     "mhy_batch_cells",
 ]
@@ -68,6 +68,7 @@ def library():
         L.mhy_kernel_name.restype = ctypes.c_char_p
         L.mhy_kernel_bytes.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.mhy_enable_kernel_timing.argtypes = [P, ctypes.c_int]
+        L.mhy_set_kernel_timing_mask.argtypes = [P, ctypes.c_uint]
         L.mhy_kernel_time.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
         L.mhy_batch_create.argtypes = [ctypes.POINTER(MhyConfig), ctypes.c_int, ctypes.POINTER(P)]
         L.mhy_batch_destroy.argtypes = [P]
@@ -165,6 +166,10 @@ class Context:
     def enable_timing(self, on=True):
         self._check(library().mhy_enable_kernel_timing(self.h, int(on)), "timing")
 
+    def set_timing_mask(self, mask):
+        """Record events only around the kernels whose bit (1 << index) is set."""
+        self._check(library().mhy_set_kernel_timing_mask(self.h, int(mask)), "timing mask")
+
     def kernel_stats(self):
         L = library()
         out = {}
@@ -174,7 +179,7 @@ class Context:
             b = ctypes.c_double()
             L.mhy_kernel_time(self.h, k, ctypes.byref(ms), ctypes.byref(n))
             L.mhy_kernel_bytes(self.h, k, ctypes.byref(b))
-            out[L.mhy_kernel_name(self.h, k).decode()] = {"avg_ms": ms.value, "launches": n.value,
+            out[L.mhy_kernel_name(self.h, k).decode()] = {"id": k, "avg_ms": ms.value, "launches": n.value,
                                                          "bytes": b.value}
         return out
 

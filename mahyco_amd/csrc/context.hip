@@ -76,6 +76,7 @@ struct mhy_ctx {
   std::vector<void*> allocs;
   // kernel timing
   bool timing = false;
+  unsigned timingMask = ~0u;  // kernels (bit 1 << KernelId) timed when timing is on
   struct Rec {
     int k;
     hipEvent_t a, b;
@@ -169,7 +170,7 @@ hipEvent_t takeEvent(mhy_ctx* x) {
 // Launch helper: optional HIP events around the kernel on the context stream.
 template <class F>
 void timed(mhy_ctx* x, int k, F&& launch) {
-  if (!x->timing) {
+  if (!x->timing || !((x->timingMask >> k) & 1u)) {
     launch();
     return;
   }
@@ -920,6 +921,12 @@ int mhy_enable_kernel_timing(mhy_ctx* x, int on) {
   collectTimes(x);
   x->timing = (on != 0);
   for (int k = 0; k < K_COUNT; ++k) x->ms[k] = 0, x->launches[k] = 0;
+  return MHY_OK;
+}
+
+int mhy_set_kernel_timing_mask(mhy_ctx* x, unsigned mask) {
+  if (!x) return MHY_ERR_ARG;
+  x->timingMask = mask;
   return MHY_OK;
 }
 
