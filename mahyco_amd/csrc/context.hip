@@ -686,6 +686,20 @@ int captureStep(mhy_ctx* x) {
   return MHY_OK;
 }
 
+// One step (without the n/n+1 swap): a replay of the captured step when
+// graphs is set, else eager launches.
+int stepOnce(mhy_ctx* x, bool graphs) {
+  if (!graphs) return runIteration(x);
+  hipGraphExec_t& g = x->graph[2 * x->cur + (x->x_then_y_n ? 1 : 0)];
+  if (!g) {
+    const int st = captureStep(x);
+    if (st != MHY_OK) return st;
+  }
+  HIPCHK(x, hipGraphLaunch(g, x->stream));
+  x->iteration++;
+  return MHY_OK;
+}
+
 int mhy_run(mhy_ctx* x, int nsteps) {
   if (!x) return MHY_ERR_ARG;
   if (x->status != MHY_OK) return x->status;
@@ -693,18 +707,8 @@ int mhy_run(mhy_ctx* x, int nsteps) {
   // events (timing) are recorded on the eager path
   const bool graphs = x->useGraph && !x->timing && nsteps >= 2;
   for (int s = 0; s < nsteps; ++s) {
-    if (graphs) {
-      hipGraphExec_t& g = x->graph[2 * x->cur + (x->x_then_y_n ? 1 : 0)];
-      if (!g) {
-        const int st = captureStep(x);
-        if (st != MHY_OK) return st;
-      }
-      HIPCHK(x, hipGraphLaunch(g, x->stream));
-      x->iteration++;
-    } else {
-      int st = runIteration(x);
-      if (st != MHY_OK) return st;
-    }
+    const int st = stepOnce(x, graphs);
+    if (st != MHY_OK) return st;
     endIteration(x);
   }
   Scalars s;
@@ -961,20 +965,16 @@ int mhy_simulate(mhy_ctx* x, const char* output_dir, int verbose) {
     lastDump = tn;
     return st;
   };
+  // the dump at t_n only reads fields that the dt phase leaves unchanged, so
+  // it comes before the step, which then runs as one graph replay
+  const bool graphs = x->useGraph && !x->timing;
   do {
-    int st = phaseDt(x);
-    if (st != MHY_OK) return st;
     Scalars s;
-    if ((st = readScalars(x, s)) != MHY_OK) return st;
+    int st = readScalars(x, s);
+    if (st != MHY_OK) return st;
     const double tn = s.t[x->cur];
     if ((st = dump(tn)) != MHY_OK) return st;
-    if ((st = phaseLagrange(x)) != MHY_OK) return st;
-    if (x->o.mode != MODE_LAGRANGE) {
-      if ((st = phaseSweep(x, 1)) != MHY_OK) return st;
-      if ((st = phaseSweep(x, 2)) != MHY_OK) return st;
-      if ((st = phaseFinal(x)) != MHY_OK) return st;
-    }
-    x->iteration++;
+    if ((st = stepOnce(x, graphs)) != MHY_OK) return st;
     if ((st = readScalars(x, s)) != MHY_OK) return st;
     if (s.err) return fail(x, s.err, "kernel status " + std::to_string(s.err));
     const double t1 = s.t[1 - x->cur];
