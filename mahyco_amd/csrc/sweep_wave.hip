@@ -168,7 +168,6 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   const bool output = in && slot >= kHalo && slot <= lastSlot - kHalo;
 
   double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
-  double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
   const double dtn = f.sc->dt[kp.cur], dt1 = f.sc->dt[kp.nxt];
   const bool needH = PP || o.limiter >= MHY_LIM_MINMODG;
   const bool mixteOpt = PP && o.pp_mixte == 1;

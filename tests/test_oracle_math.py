@@ -7,6 +7,8 @@ mathematics fixes rather than against a retyped formula:
   van Leer = harmonic mean);
 * generalised limiters (remap/UtilesRemap.cc:25-72) reducing to those forms
   on uniform widths, and their cap by the one-sided slopes on any widths;
+  arithmeticG as the exact centroid derivative of the quadratic through
+  three cell means on any widths, vanLeerG as its harmonic counterpart;
 * INT2Y (remap/UtilesRemap.cc:177-188) against Gauss quadrature;
 * the PenteBorne profile (UtilesRemap-Impl.h:142-316): mean preservation
   over a whole-cell sweep, boundedness by the neighbours, exactness on
@@ -116,6 +118,51 @@ def test_generalised_limiters_are_capped_and_monotone_on_any_widths(orc, glim):
         assert abs(g) <= cap * (1 + 1e-14)
         if g != 0.0:
             assert np.sign(g) == np.sign(phip - phim)
+
+
+def _quadratic_means(rng):
+    """Three cells of random widths and the means of a gently curved
+    quadratic over them; face gradients between centroids as the
+    reference forms them (gradPhiFace = difference / centroid distance)."""
+    hm, h0, hp = rng.uniform(0.5, 2.0, 3)
+    e = np.array([0.0, hm, hm + h0, hm + h0 + hp]) + rng.uniform(-2, 2)
+    A, B = rng.uniform(-0.05, 0.05), rng.choice([-1.0, 1.0]) * rng.uniform(1.0, 2.0)
+    P = lambda x: A * x ** 3 / 3 + B * x ** 2 / 2  # noqa: E731
+    ym, y0, yp = [(P(e[k + 1]) - P(e[k])) / (e[k + 1] - e[k]) for k in range(3)]
+    c = 0.5 * (e[:-1] + e[1:])
+    gm, gp = (y0 - ym) / (c[1] - c[0]), (yp - y0) / (c[2] - c[1])
+    return (hm, h0, hp), (ym, y0, yp), (gm, gp), 2 * A * c[1] + B
+
+
+def test_arithmetic_g_is_the_quadratic_derivative_on_any_widths(orc):
+    """arithmeticG (remap/UtilesRemap.cc:55-59) weights the two face
+    gradients so that it returns the exact derivative, at the centre cell's
+    centroid, of the quadratic whose cell means are the three given ones, on
+    any widths.  The gradients differ by less than the simple-slope cap
+    (formula 10), which therefore does not bind."""
+    rng = np.random.default_rng(21)
+    for _ in range(300):
+        (hm, h0, hp), (ym, y0, yp), (gm, gp), exact = _quadratic_means(rng)
+        g = orc.probe_gradient(LIM["arithmeticG"], gp, gm, y0, yp, ym, h0, hp, hm)
+        assert g == pytest.approx(exact, rel=1e-12, abs=1e-13)
+
+
+def test_van_leer_g_is_the_reciprocal_of_the_weighted_mean_of_reciprocals(orc):
+    """vanLeerG (remap/UtilesRemap.cc:45-53) is the harmonic counterpart of
+    arithmeticG: 1 / vanLeerG(g+, g-) = arithmeticG(1/g+, 1/g-) with the same
+    widths, the plus-side weight on the minus-side slope.  Slopes of one
+    sign within a factor 1.25 of each other keep the cap from binding."""
+    rng = np.random.default_rng(22)
+    for _ in range(300):
+        hm, h0, hp = rng.uniform(0.5, 2.0, 3)
+        s = rng.choice([-1.0, 1.0])
+        gm = s * rng.uniform(1.0, 2.0)
+        gp = gm * rng.uniform(0.8, 1.25)
+        ym, y0 = 0.0, s * 1.0
+        yp = y0 + s  # the data trend gives the sign
+        v = orc.probe_gradient(LIM["vanLeerG"], gp, gm, y0, yp, ym, h0, hp, hm)
+        a = orc.probe_gradient(LIM["arithmeticG"], 1.0 / gp, 1.0 / gm, y0, yp, ym, h0, hp, hm)
+        assert v * a == pytest.approx(1.0, rel=1e-12)
 
 
 def test_int2y_is_the_integral_of_the_segment():
