@@ -9,6 +9,8 @@ mathematics fixes rather than against a retyped formula:
   on uniform widths, and their cap by the one-sided slopes on any widths;
   arithmeticG as the exact centroid derivative of the quadratic through
   three cell means on any widths, vanLeerG as its harmonic counterpart;
+* the boundary inflow flux (remap/BoundaryFluxes.cc) by the mass balance of
+  the reference's UnitTestCase;
 * INT2Y (remap/UtilesRemap.cc:177-188) against Gauss quadrature;
 * the PenteBorne profile (UtilesRemap-Impl.h:142-316): mean preservation
   over a whole-cell sweep, boundedness by the neighbours, exactness on
@@ -499,3 +501,29 @@ def test_x_and_y_runs_mirror(opts):
     finally:
         ox.close()
         oy.close()
+
+
+def test_boundary_inflow_mass_balance():
+    """BoundaryFluxes (remap/BoundaryFluxes.cc:14-120) on the reference's
+    UnitTestCase: a uniform flow u = 1 enters through the left edge, where
+    the region x < 0.4 holds rho = 1, and no flux leaves (closed faces
+    elsewhere).  Interior face fluxes cancel, so each step's total mass grows
+    by exactly rho * u * (ny * dy) * dt_n: the inflow uses the face value of
+    the boundary cell's opposite face (rho = 1 there) and the remap's dt_n.
+    A flux through the wrong face, a sign error or dt_{n+1} fails it."""
+    c = configs.default_config()
+    c.test_case, c.schema, c.nbmat = abi.CASE["UnitTestCase"], abi.SCHEMA_VNR, 1
+    c.nx, c.ny, c.dx, c.dy = 40, 4, 0.025, 0.025
+    c.limiter, c.pseudo_centree, c.abort_on_energy = LIM["minmod"], 0, 0
+    o = refcpu.Oracle(c)
+    try:
+        m0 = o.get("cell_mass").sum()
+        for _ in range(10):
+            dtn = o.time_state().deltat_n
+            assert o.step(1) == abi.MHY_OK
+            m1 = o.get("cell_mass").sum()
+            # the increment is ~1e-10 of a total of ~0.04: compare to 1e-6
+            assert (m1 - m0) == pytest.approx(1.0 * 1.0 * c.ny * c.dy * dtn, rel=1e-6)
+            m0 = m1
+    finally:
+        o.close()
