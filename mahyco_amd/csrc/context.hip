@@ -968,10 +968,15 @@ int mhy_simulate(mhy_ctx* x, const char* output_dir, int verbose) {
   // the dump at t_n only reads fields that the dt phase leaves unchanged, so
   // it comes before the step, which then runs as one graph replay
   const bool graphs = x->useGraph && !x->timing;
-  do {
-    Scalars s;
-    int st = readScalars(x, s);
+  // the time scalars read after a step are those the next step starts from
+  // (nothing runs on the device in between), so one read per step serves both
+  Scalars s;
+  {
+    const int st = readScalars(x, s);
     if (st != MHY_OK) return st;
+  }
+  do {
+    int st;
     const double tn = s.t[x->cur];
     if ((st = dump(tn)) != MHY_OK) return st;
     if ((st = stepOnce(x, graphs)) != MHY_OK) return st;
