@@ -322,6 +322,9 @@ struct ref_ctx {
   void computeFluxPP(bool pureVariant, const VE& grad, const VE& phi, const VE& phip, const VE& phim, double h0,
                      double hp, double hm, double fnv, double dt, int type, double thr, int complet, double dualv,
                      int calcDual, VE* pFlux, VE* pFluxDual) const;
+  void gradPhiCell(const VE& gp, const VE& gm, const VE& phi, const VE& phip, const VE& phim, const int* mix3,
+                   const int* pure3, double h0, double hp, double hm, double Fm, double Fp, double fdual, double dt,
+                   VE* grad, VE* Ar, VE* Av, VE* dual) const;
   VE computeRemapFlux(double fnv, const V2& fn, double fl, const VE& phiFace, const V2& ofn, const V2& exy,
                       double dt) const;
   VE computeBoundaryFluxes(int proj, long c, const V2& exy) const;
@@ -1469,7 +1472,6 @@ void ref_ctx::computeGradPhi(int sweep) {
   auto& gPF = (sweep == 1) ? gradPhiFace1 : gradPhiFace2;
   auto& gP = (sweep == 1) ? gradPhi1 : gradPhi2;
   auto& DPF = (sweep == 1) ? DualphiFlux1 : DualphiFlux2;
-  const int calcDual = (o.methode_flux_masse == 2) ? 1 : 0;
 #pragma omp parallel for
   for (long c = 0; c < mesh.nbCells; ++c) {
     auto fc = mesh.facesOfCell(c);
@@ -1485,30 +1487,49 @@ void ref_ctx::computeGradPhi(int sweep) {
     }
     if (cm == -1) cm = c;
     if (cp == -1) cp = c;
-    bool vp = (o.pp_mixte == 1) && (mixte[c] == 0 && mixte[cp] == 0 && mixte[cm] == 0 && pure[c] == pure[cp] &&
-                                    pure[c] == pure[cm]);
-    int lim = o.limiter;
-    if ((o.pp == 1) && vp) lim = o.limiter_pure;
-    if (xdir)
-      gP[c] = computeAndLimitGradPhi(lim, gPF[fp], gPF[fm], Phi[c], Phi[cp], Phi[cm], HvLagrange[c], HvLagrange[cp],
-                                     HvLagrange[cm]);
-    else
-      gP[c] = computeAndLimitGradPhi(lim, gPF[fm], gPF[fp], Phi[c], Phi[cp], Phi[cm], HvLagrange[c], HvLagrange[cp],
-                                     HvLagrange[cm]);
+    const int mix3[3] = {mixte[c], mixte[cp], mixte[cm]}, pure3[3] = {pure[c], pure[cp], pure[cm]};
+    // x: (plus face, minus face) gradients; y: (minus, plus), as the reference passes them
+    const VE& ga = xdir ? gPF[fp] : gPF[fm];
+    const VE& gb = xdir ? gPF[fm] : gPF[fp];
+    double Fm = 0.0, Fp = 0.0, fdual = 0.0;
     if (o.pp == 1) {
       const V2 exy = xdir ? V2{1.0, 0.0} : V2{0.0, 1.0};
-      double Fm = dot2(outerFaceNormal[c][sm], exy) * faceNormalVelocity[fm];
-      double Fp = dot2(outerFaceNormal[c][sp], exy) * faceNormalVelocity[fp];
-      double fdual = xdir ? 0.5 * (faceNormalVelocity[fm] + faceNormalVelocity[fp])
-                          : 0.5 * (faceNormalVelocity[fm] + faceNormalVelocity[fp]);
-      VE bidon;
-      computeFluxPP(vp, gP[c], Phi[c], Phi[cp], Phi[cm], HvLagrange[c], HvLagrange[cp], HvLagrange[cm], Fm, deltat_n,
-                    0, o.threshold, o.pp_complet, fdual, calcDual, &deltaPhiFaceAr[c], &DPF[c]);
-      const double w = xdir ? (faceLength[fm] + faceLength[fp]) * 0.5 : (faceLength[fm] + faceLength[fp]) * 0.5;
-      for (int e = 0; e < NEQ; ++e) DPF[c][e] *= w;
-      computeFluxPP(vp, gP[c], Phi[c], Phi[cp], Phi[cm], HvLagrange[c], HvLagrange[cp], HvLagrange[cm], Fp, deltat_n,
-                    1, o.threshold, o.pp_complet, fdual, calcDual, &deltaPhiFaceAv[c], &bidon);
+      Fm = dot2(outerFaceNormal[c][sm], exy) * faceNormalVelocity[fm];
+      Fp = dot2(outerFaceNormal[c][sp], exy) * faceNormalVelocity[fp];
+      fdual = 0.5 * (faceNormalVelocity[fm] + faceNormalVelocity[fp]);
     }
+    gradPhiCell(ga, gb, Phi[c], Phi[cp], Phi[cm], mix3, pure3, HvLagrange[c], HvLagrange[cp], HvLagrange[cm], Fm,
+                Fp, fdual, deltat_n, &gP[c], &deltaPhiFaceAr[c], &deltaPhiFaceAv[c], &DPF[c]);
+    if (o.pp == 1) {
+      const double w = (faceLength[fm] + faceLength[fp]) * 0.5;
+      for (int e = 0; e < NEQ; ++e) DPF[c][e] *= w;
+    }
+  }
+}
+
+// The cell stage of computeGradPhi{1,2} (PhaseRemap1.cc:134-320,
+// PhaseRemap2.cc:126-318) for one cell c with its plus / minus neighbours:
+// the PenteBorne-mixte test of a pure neighbourhood (c and both neighbours
+// not mixed and holding the same material; flags in the order c, plus,
+// minus), the limiter it selects, the limited gradient, and with PenteBorne
+// the back (Ar, type 0) and front (Av, type 1) swept fluxes of
+// computeFluxPP or, in a pure neighbourhood, computeFluxPPPure, plus the PB
+// dual flux (before its face-length weight).
+void ref_ctx::gradPhiCell(const VE& gp, const VE& gm, const VE& phi, const VE& phip, const VE& phim,
+                          const int* mix3, const int* pure3, double h0, double hp, double hm, double Fm, double Fp,
+                          double fdual, double dt, VE* grad, VE* Ar, VE* Av, VE* dual) const {
+  const bool vp = (o.pp_mixte == 1) &&
+                  (mix3[0] == 0 && mix3[1] == 0 && mix3[2] == 0 && pure3[0] == pure3[1] && pure3[0] == pure3[2]);
+  int lim = o.limiter;
+  if ((o.pp == 1) && vp) lim = o.limiter_pure;
+  *grad = computeAndLimitGradPhi(lim, gp, gm, phi, phip, phim, h0, hp, hm);
+  if (o.pp == 1) {
+    const int calcDual = (o.methode_flux_masse == 2) ? 1 : 0;
+    VE bidon;
+    computeFluxPP(vp, *grad, phi, phip, phim, h0, hp, hm, Fm, dt, 0, o.threshold, o.pp_complet, fdual, calcDual, Ar,
+                  dual);
+    computeFluxPP(vp, *grad, phi, phip, phim, h0, hp, hm, Fp, dt, 1, o.threshold, o.pp_complet, fdual, calcDual, Av,
+                  &bidon);
   }
 }
 
@@ -2151,6 +2172,17 @@ double ref_probe_flux_pp(ref_ctx* r, int lim, double grad, double phi, double ph
   r->computeFluxPP(false, g, v, vp, vm, h0, hp, hm, fnv, dt, type, r->o.threshold, 0, 0.0, 0, &flux, &dual);
   r->o.limiter = keep;
   return flux[0];
+}
+
+int ref_probe_cell_pp(ref_ctx* r, const double* gp, const double* gm, const double* phi, const double* phip,
+                      const double* phim, const int* mix3, const int* pure3, double h0, double hp, double hm,
+                      double Fm, double Fp, double fdual, double dt, double* grad, double* ar, double* av,
+                      double* dual) {
+  VE vgp, vgm, v, vp, vm, g = ezero(), a = ezero(), b = ezero(), d = ezero();
+  for (int i = 0; i < NEQ; ++i) vgp[i] = gp[i], vgm[i] = gm[i], v[i] = phi[i], vp[i] = phip[i], vm[i] = phim[i];
+  r->gradPhiCell(vgp, vgm, v, vp, vm, mix3, pure3, h0, hp, hm, Fm, Fp, fdual, dt, &g, &a, &b, &d);
+  for (int i = 0; i < NEQ; ++i) grad[i] = g[i], ar[i] = a[i], av[i] = b[i], dual[i] = d[i];
+  return NEQ;
 }
 
 int ref_step(ref_ctx* r) {

@@ -41,6 +41,18 @@ double ref_probe_int2y(double X, double x0, double y0, double x1, double y1);
 double ref_probe_flux_pp(ref_ctx* r, int lim, double grad, double phi, double phip, double phim, double h0, double hp,
                          double hm, double fnv, double dt, int type);
 
+/* The cell stage of computeGradPhi (remap/PhaseRemap1.cc:134-320) on one
+ * cell: arrays of the 13 vector components; gp / gm as the reference passes
+ * them to computeAndLimitGradPhi; mix3 / pure3 = mixte / pure flags of the
+ * cell, its plus and its minus neighbour; Fm / Fp the oriented normal
+ * velocities of the back / front faces, fdual the PB dual velocity.
+ * Outputs the limited gradient, the back (Ar) and front (Av) PenteBorne
+ * fluxes and the PB dual flux.  Returns the component count (13). */
+int ref_probe_cell_pp(ref_ctx* r, const double* gp, const double* gm, const double* phi, const double* phip,
+                      const double* phim, const int* mix3, const int* pure3, double h0, double hp, double hm,
+                      double Fm, double Fp, double fdual, double dt, double* grad, double* ar, double* av,
+                      double* dual);
+
 /* The third-order face value of one component (remap/UtilesRemap.cc:197-320):
  * y, h = means and widths of the cells c-3 .. c+2 around the face, vdt = face
  * normal velocity times dt. */

@@ -52,6 +52,9 @@ def lib():
         L.ref_probe_int2y.restype = d
         L.ref_probe_flux_pp.argtypes = [ctypes.c_void_p, ctypes.c_int] + [d] * 9 + [ctypes.c_int]
         L.ref_probe_flux_pp.restype = d
+        dp, ip = ctypes.POINTER(d), ctypes.POINTER(ctypes.c_int)
+        L.ref_probe_cell_pp.argtypes = [ctypes.c_void_p] + [dp] * 5 + [ip] * 2 + [d] * 7 + [dp] * 4
+        L.ref_probe_cell_pp.restype = ctypes.c_int
         L.ref_probe_flux_o3.argtypes = [ctypes.c_void_p, ctypes.POINTER(d), ctypes.POINTER(d), d]
         L.ref_probe_flux_o3.restype = d
         _lib = L
@@ -123,6 +126,24 @@ class Oracle:
     def probe_flux_pp(self, lim, grad, phi, phip, phim, h0, hp, hm, fnv, dt, face):
         """PenteBorne swept flux of one component; face 0 = back, 1 = front."""
         return lib().ref_probe_flux_pp(self.h, lim, grad, phi, phip, phim, h0, hp, hm, fnv, dt, face)
+
+    def probe_cell_pp(self, gp, gm, phi, phip, phim, mixte, pure, h0, hp, hm, fm, fp, fdual, dt):
+        """The cell stage of computeGradPhi on one cell (13-component vectors;
+        mixte / pure: flags of the cell, its plus and its minus neighbour).
+        Returns (limited gradient, Ar, Av, PB dual flux)."""
+        def arr(v):
+            a = np.zeros(13, dtype=np.float64)
+            a[:len(v)] = v
+            return a
+        ins = [arr(v) for v in (gp, gm, phi, phip, phim)]
+        outs = [np.zeros(13, dtype=np.float64) for _ in range(4)]
+        P = ctypes.POINTER(ctypes.c_double)
+        mix3 = (ctypes.c_int * 3)(*[int(v) for v in mixte])
+        pure3 = (ctypes.c_int * 3)(*[int(v) for v in pure])
+        lib().ref_probe_cell_pp(self.h, *[a.ctypes.data_as(P) for a in ins], mix3, pure3, float(h0), float(hp),
+                                float(hm), float(fm), float(fp), float(fdual), float(dt),
+                                *[a.ctypes.data_as(P) for a in outs])
+        return tuple(outs)
 
     def probe_flux_o3(self, y, h, vdt):
         """Third-order face value of one component (remap/UtilesRemap.cc:197-320);

@@ -15,6 +15,12 @@ mathematics fixes rather than against a retyped formula:
 * the PenteBorne profile (UtilesRemap-Impl.h:142-316): mean preservation
   over a whole-cell sweep, boundedness by the neighbours, exactness on
   linear data, the flat (extremum) branch, back/front mirror symmetry;
+* PenteBorne "mixte" (UtilesRemap-Impl.h:317-489 through
+  PhaseRemap1.cc:134-320): the pure-neighbourhood test (a mixed or
+  other-material neighbour on either side switches it off), the pure-cell
+  limiter it selects, and the mass flux of a pure neighbourhood as the swept
+  integral of the density profile (exact for a linear density) instead of
+  density times swept volume;
 * whole runs with options no fixture uses (G-limiters, van Leer, PenteBorne
   mixte and complet, A2/PB dual fluxes, three materials): per-material mass
   conservation, volume fractions in [0,1] summing to 1, a uniform state left
@@ -527,3 +533,207 @@ def test_boundary_inflow_mass_balance():
             m0 = m1
     finally:
         o.close()
+
+
+# ------------------------------------------------------- PenteBorne "mixte"
+# With projectionPlateauPenteMixte, computeGradPhi{1,2}
+# (remap/PhaseRemap1.cc:134-320, PhaseRemap2.cc:126-318) tests each cell's
+# neighbourhood along the sweep: a cell that is not mixed, whose two
+# neighbours are not mixed either and hold the same material, takes the
+# pure-cell limiter and computeFluxPPPure (remap/UtilesRemap-Impl.h:317-489),
+# whose mass flux is the swept integral of the density profile itself; any
+# other cell keeps the main limiter and computeFluxPP, whose mass flux is the
+# cell density times the swept volume.  The expected fluxes below are built
+# from the single-component swept integral and limited slope pinned above
+# (mean preservation, bounds, linear exactness, extrema, mirror symmetry).
+
+NEQ = 13
+M2 = 2  # materials of the probe problem: components f 0-1, rho 2-3, e 4-5, q 9
+
+
+def _pp_oracle(limiter, limiter_pure, mixte=1, complet=0):
+    c = configs.riemann(8, 0.5, 1.0, 1.0, 0.125, 0.1, 1.4, 1.67)
+    c.pp, c.pp_mixte, c.pp_complet = 1, mixte, complet
+    c.limiter, c.limiter_pure = LIM[limiter], LIM[limiter_pure]
+    return refcpu.Oracle(c)
+
+
+def _derive(raw, phi, pure_variant, complet, m=M2, thr=1e-16):
+    """Mass, energy, momentum and pseudo components from the per-component
+    swept integrals raw (the derivations of computeFluxPP and
+    computeFluxPPPure)."""
+    F = raw.copy()
+    if complet:
+        sfv = sum(raw[k] for k in range(m))
+        if abs(sfv) <= thr:
+            return np.zeros(NEQ)
+        sfm = 0.0
+        for k in range(m):
+            if not pure_variant:
+                F[m + k] = (raw[m + k] / sfv) * raw[k]
+            F[2 * m + k] = (raw[2 * m + k] / sfv) * F[m + k]
+            sfm += F[m + k]
+        for j in range(3):
+            F[3 * m + j] = (raw[3 * m + j] / sfv) * sfm
+        F[3 * m + 3] = phi[3 * m + 3] * sfv
+        return F
+    sfm = sfv = 0.0
+    for k in range(m):
+        if not pure_variant:
+            F[m + k] = phi[m + k] * raw[k]
+        F[2 * m + k] = phi[2 * m + k] * F[m + k]
+        sfm += F[m + k]
+        sfv += raw[k]
+    for j in range(3):
+        F[3 * m + j] = phi[3 * m + j] * sfm
+    F[3 * m + 3] = phi[3 * m + 3] * sfv
+    return F
+
+
+def _expected_cell(orc, lim, pure_variant, complet, gp, gm, phi, phip, phim, h0, hp, hm, fm, fp, dt):
+    g = np.array([orc.probe_gradient(LIM[lim], gp[i], gm[i], phi[i], phip[i], phim[i], h0, hp, hm)
+                  for i in range(NEQ)])
+    out = []
+    for face, v in ((0, fm), (1, fp)):
+        raw = np.array([orc.probe_flux_pp(LIM[lim], g[i], phi[i], phip[i], phim[i], h0, hp, hm, v, dt, face)
+                        for i in range(NEQ)])
+        F = _derive(raw, phi, pure_variant, complet)
+        if 0.5 * (v + abs(v)) * dt == 0.0:
+            F = np.zeros(NEQ)
+        out.append(F)
+    return g, out[0], out[1]
+
+
+def _pp_state(rng, mats):
+    """Phi in the PenteBorne form for a cell holding the materials mats
+    (fractions summing to 1), with random densities, mixture energy and
+    pseudo."""
+    phi = np.zeros(NEQ)
+    fr = rng.dirichlet(np.ones(len(mats))) if len(mats) > 1 else np.ones(1)
+    e = rng.uniform(0.5, 3.0)
+    for k, f in zip(mats, fr):
+        phi[k] = f
+        phi[M2 + k] = rng.uniform(0.1, 1.0)
+        phi[2 * M2 + k] = e
+    phi[3 * M2 + 3] = rng.uniform(0.0, 0.01)
+    return phi
+
+
+def _pp_inputs(rng, mats3):
+    """Cell, plus and minus states, widths, face gradients (as the reference
+    passes them to computeAndLimitGradPhi along x), face velocities, dt."""
+    phi, phip, phim = (_pp_state(rng, m) for m in mats3)
+    h0, hp, hm = rng.uniform(0.5, 2.0, 3)
+    gp = (phip - phi) / (0.5 * (h0 + hp))
+    gm = (phi - phim) / (0.5 * (h0 + hm))
+    fm, fp = rng.uniform(-0.3, 0.6, 2)
+    return gp, gm, phi, phip, phim, h0, hp, hm, fm, fp, 0.4 * h0
+
+
+LIM_PAIRS = [("superBee", "minmod"), ("minmod", "arithmeticG"), ("vanLeer", "superBeeG"), ("superBee", "ultrabeeG")]
+
+
+@pytest.mark.parametrize("complet", [0, 1], ids=["simple", "complet"])
+@pytest.mark.parametrize("lim,lim_pure", LIM_PAIRS, ids=[f"{a}-{b}" for a, b in LIM_PAIRS])
+def test_pp_mixte_pure_neighbourhood_takes_the_pure_limiter_and_density_profile(lim, lim_pure, complet):
+    orc = _pp_oracle(lim, lim_pure, complet=complet)
+    rng = np.random.default_rng(41 + complet)
+    differs = 0
+    for _ in range(60):
+        mat = int(rng.integers(0, M2))
+        ins = _pp_inputs(rng, ([mat], [mat], [mat]))
+        g, ar, av, _ = orc.probe_cell_pp(*ins[:5], (0, 0, 0), (mat, mat, mat), *ins[5:10], 0.0, ins[10])
+        wg, war, wav = _expected_cell(orc, lim_pure, True, complet, *ins)
+        np.testing.assert_allclose(g, wg, rtol=1e-14, atol=0)
+        np.testing.assert_allclose(ar, war, rtol=1e-13, atol=1e-300)
+        np.testing.assert_allclose(av, wav, rtol=1e-13, atol=1e-300)
+        # the main limiter with the density-times-volume mass flux: what a
+        # cell outside a pure neighbourhood gets
+        _, nar, nav = _expected_cell(orc, lim, False, complet, *ins)
+        differs += not (np.allclose(ar, nar, rtol=1e-12) and np.allclose(av, nav, rtol=1e-12))
+    orc.close()
+    assert differs >= 10  # the data separate the two variants
+
+
+# (mixte flags, pure flags) of (cell, plus neighbour, minus neighbour)
+NEIGHBOURHOODS = {
+    "all_pure_mat0": ((0, 0, 0), (0, 0, 0)),
+    "all_pure_mat1": ((0, 0, 0), (1, 1, 1)),
+    "cell_mixed": ((1, 0, 0), (-1, 0, 0)),
+    "plus_mixed": ((0, 1, 0), (0, -1, 0)),
+    "minus_mixed": ((0, 0, 1), (0, 0, -1)),
+    "plus_other_material": ((0, 0, 0), (0, 1, 0)),
+    "minus_other_material": ((0, 0, 0), (0, 0, 1)),
+    "both_other_material": ((0, 0, 0), (1, 0, 0)),
+}
+
+
+def _mats(mix, pure):
+    return [[0, 1] if mx else [p] for mx, p in zip(mix, pure)]
+
+
+@pytest.mark.parametrize("name", sorted(NEIGHBOURHOODS))
+def test_pp_mixte_pure_neighbourhood_detection(name):
+    mix, pure = NEIGHBOURHOODS[name]
+    vp = all(v == 0 for v in mix) and pure[0] == pure[1] == pure[2]
+    orc = _pp_oracle("superBee", "minmod")
+    rng = np.random.default_rng(43)
+    for _ in range(40):
+        ins = _pp_inputs(rng, _mats(mix, pure))
+        g, ar, av, _ = orc.probe_cell_pp(*ins[:5], mix, pure, *ins[5:10], 0.0, ins[10])
+        wg, war, wav = _expected_cell(orc, "minmod" if vp else "superBee", vp, 0, *ins)
+        np.testing.assert_allclose(g, wg, rtol=1e-14, atol=0)
+        np.testing.assert_allclose(ar, war, rtol=1e-13, atol=1e-300)
+        np.testing.assert_allclose(av, wav, rtol=1e-13, atol=1e-300)
+    orc.close()
+
+
+def test_pp_without_mixte_ignores_pure_neighbourhoods():
+    orc = _pp_oracle("superBee", "minmod", mixte=0)
+    rng = np.random.default_rng(44)
+    for _ in range(40):
+        ins = _pp_inputs(rng, ([0], [0], [0]))
+        g, ar, av, _ = orc.probe_cell_pp(*ins[:5], (0, 0, 0), (0, 0, 0), *ins[5:10], 0.0, ins[10])
+        wg, war, wav = _expected_cell(orc, "superBee", False, 0, *ins)
+        np.testing.assert_allclose(ar, war, rtol=1e-13, atol=1e-300)
+        np.testing.assert_allclose(av, wav, rtol=1e-13, atol=1e-300)
+    orc.close()
+
+
+@pytest.mark.parametrize("lim_pure", ["minmod", "superBee", "vanLeer"])
+def test_pp_pure_mass_flux_is_the_exact_integral_of_a_linear_density(lim_pure):
+    """One material, equal widths, density and mixture energy linear across
+    the three cells: in a pure neighbourhood the swept mass is the exact
+    integral of the density line, v*rho + slope*(h*v - v^2)/2 through the
+    front face and v*rho - slope*(h*v - v^2)/2 through the back face, and the
+    energy flux is the cell's specific energy times it.  The same cell with
+    the mixte option off carries rho times the swept volume."""
+    rng = np.random.default_rng(45)
+    pure_orc = _pp_oracle("superBee", lim_pure)
+    plain_orc = _pp_oracle("superBee", lim_pure, mixte=0)
+    for _ in range(50):
+        h = rng.uniform(0.3, 2.0)
+        rho, srho = rng.uniform(0.5, 1.0), rng.uniform(-0.2, 0.2)
+        e, q = rng.uniform(1.0, 2.0), rng.uniform(0.0, 0.01)
+        phis = []
+        for dx in (0.0, h, -h):  # cell, plus, minus
+            v = np.zeros(NEQ)
+            v[0], v[M2], v[2 * M2], v[3 * M2 + 3] = 1.0, rho + srho * dx, e, q
+            phis.append(v)
+        phi, phip, phim = phis
+        gp, gm = (phip - phi) / h, (phi - phim) / h
+        vdt = rng.uniform(0.01, 1.0) * h
+        args = (gp, gm, phi, phip, phim, (0, 0, 0), (0, 0, 0), h, h, h, 1.0, 1.0, 0.0, vdt)
+        _, ar, av, _ = pure_orc.probe_cell_pp(*args)
+        front = vdt * rho + srho * (h * vdt - vdt * vdt) / 2
+        back = vdt * rho - srho * (h * vdt - vdt * vdt) / 2
+        assert av[0] == pytest.approx(vdt, rel=1e-13) and ar[0] == pytest.approx(vdt, rel=1e-13)
+        assert av[M2] == pytest.approx(front, rel=1e-12)
+        assert ar[M2] == pytest.approx(back, rel=1e-12)
+        assert av[2 * M2] == pytest.approx(e * front, rel=1e-12)
+        assert av[3 * M2 + 3] == pytest.approx(q * vdt, rel=1e-12)
+        _, par, pav, _ = plain_orc.probe_cell_pp(*args)
+        assert pav[M2] == pytest.approx(rho * vdt, rel=1e-13)
+        assert par[M2] == pytest.approx(rho * vdt, rel=1e-13)
+    pure_orc.close()
+    plain_orc.close()
