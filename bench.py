@@ -5,13 +5,16 @@ Lagrange phase, two remap sweeps with the dual remap, final remap + EOS)
 over the whole mesh; the metric is cell-updates/s = cells x steps / wall
 time of the timed steps, with the state resident in HBM.
 
-Default workload: the two-material Riemann problem on 2048^2 with a
-perturbed interface, superBee + PenteBorne, FP64.  --config c4 runs the
-three-material layout in the mixed FP32 / FP64 form with a per-field error
-report against the FP64 form after the run; --config c5 runs the 512-problem
-ensemble batched (mixed form by default) against the plain FP64 form;
---config c1 runs the reference's own test input (Sod X on 200x5), whose
-1000 cells leave the step bound by launch latency.
+Default workload: the point explosion on 4096^2 (one material, minmod
+order 2) in the plain FP64 form, the bandwidth-roofline run.  The same line
+carries, under "ensemble", the 512-problem ensemble (sides 64..1024, seed
+20261204) stepped batched in the mixed FP32 / FP64 form against the same
+problems stepped one after another in the plain FP64 form (--no-ensemble
+skips it).  --config c2 runs the two-material Riemann problem on 2048^2
+(superBee + PenteBorne); --config c4 runs the three-material layout in the
+mixed form with a per-field error report against the FP64 form; --config c5
+runs the ensemble alone; --config c1 runs the reference's own test input
+(Sod X on 200x5), whose 1000 cells leave the step bound by launch latency.
 
 Prints one JSON line.  --gpus N > 1 exits: the library runs on one GPU.
 """
@@ -29,6 +32,12 @@ HBM_PEAK = 8.0e12
 
 
 PRECISION_NAME = {0: "FP64", 1: "mixed FP32/FP64"}
+
+
+def progress(msg):
+    """One line on stderr at each phase boundary (stdout carries only the
+    JSON result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 DTYPE = {0: "f64", 1: "f32+f64"}
 
 
@@ -75,8 +84,10 @@ def cpu_baseline(cfg, seconds_budget):
     from oracle import refcpu
     cores = os.cpu_count() or 1
     refcpu.set_threads(cores)
+    progress(f"cpu baseline: oracle set-up {cfg.nx}x{cfg.ny}, {cores} threads")
     orc = refcpu.Oracle(cfg)
     orc.step(1)  # warm-up step (first-touch of the arrays)
+    progress("cpu baseline: timed steps")
     t0 = time.perf_counter()
     n = 0
     while True:
@@ -88,7 +99,7 @@ def cpu_baseline(cfg, seconds_budget):
     orc.close()
     cells = cfg.nx * cfg.ny
     return {"value": cells * n / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"CPU oracle (reference-semantics restatement, g++ -O2 -fopenmp), same {cfg.nx}x{cfg.ny} "
+            "sample": f"CPU oracle (reference-semantics restatement, g++ -O3 -fopenmp), same {cfg.nx}x{cfg.ny} "
                       f"mesh, {n} timed steps after 1 warm-up step"}
 
 
@@ -141,51 +152,69 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
 
 
-def bench_ensemble(args):
+def ensemble_rates(args):
     """C5: 512 two-material Riemann problems of uneven sizes stepped together
-    (one launch per kernel over all problems, HIP graphs), in the mixed form
-    unless --precision fp64, against the same problems run one after another
-    with the plain single-problem FP64 path."""
+    (one launch per kernel over all problems, two captured HIP graphs), in
+    the mixed form unless --precision fp64, against the same problems stepped
+    one after another with the plain single-problem FP64 path.  Both legs
+    replay captured HIP graphs in their timed steps: the batch captures its
+    two graphs in its warm-up, and each plain context runs two untimed steps
+    first, which capture its two graphs (one per step parity)."""
     import mahyco_amd
     from mahyco_amd import configs
     prec = 1 if args.precision in (None, "mixed") else 0
     cfgs = configs.ensemble(n=args.ensemble)
     for c in cfgs:
         c.precision = prec
+    progress(f"ensemble: {len(cfgs)} problems, batched set-up")
     batch = mahyco_amd.Batch(cfgs)
     cells = batch.cells
     batch.run(max(args.warmup, 12))  # the first 10 steps also tune the launch widths
+    progress("ensemble: batched timed steps")
     t0 = time.perf_counter()
     batch.run(args.steps)  # ends with a stream synchronisation
     wall = time.perf_counter() - t0
     batch.close()
     value = cells * args.steps / wall
-    # the same problems in the plain FP64 form, one at a time (bounded number
-    # of steps)
     plain_steps = max(1, min(args.steps, args.plain_steps))
     plain_t = 0.0
-    for c in cfgs:
+    progress("ensemble: plain FP64 problems one after another")
+    for n, c in enumerate(cfgs):
+        if n and n % 128 == 0:
+            progress(f"ensemble: plain problem {n}")
         c.precision = 0
         ctx = mahyco_amd.Context(c)
-        ctx.run(1)
+        ctx.run(2)  # captures the graphs of both step parities
         t1 = time.perf_counter()
         ctx.run(plain_steps)
         plain_t += time.perf_counter() - t1
         ctx.close()
     plain = cells * plain_steps / plain_t
+    return {"workload": f"ensemble of {len(cfgs)} two-material Riemann problems, sides 64..1024 log-uniform "
+                        f"(seed 20261204), minmod, batched, {PRECISION_NAME[prec]}",
+            "problems": len(cfgs), "cells": cells, "dtype": DTYPE[prec], "steps": args.steps,
+            "value": value, "unit": "cell-updates/s", "ms_per_step": wall / args.steps * 1e3,
+            "plain_form": {"value": plain, "unit": "cell-updates/s", "steps": plain_steps,
+                           "note": "same problems, FP64, one context after another (mhy_run per problem, "
+                                   "graphs captured before the timed steps)"},
+            "batched_over_plain": value / plain}
+
+
+def bench_ensemble(args):
+    """--config c5: the ensemble comparison as the line itself."""
+    r = ensemble_rates(args)
+    prec = 1 if r["dtype"] == DTYPE[1] else 0
     B = algorithmic_bytes_per_cell(2, prec)
     out = {
-        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": DTYPE[prec], "data": "synthetic",
-        "config": {"workload": f"ensemble of {len(cfgs)} two-material Riemann problems, sides 64..1024 "
-                               f"log-uniform (seed 20261204), minmod, batched, {PRECISION_NAME[prec]}",
-                   "problems": len(cfgs), "cells": cells, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "whole step (batched)", "achieved": B * value / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B * value / HBM_PEAK, "traffic": None},
-        "plain_form": {"value": plain, "unit": "cell-updates/s", "steps": plain_steps,
-                       "note": "same problems, FP64, one context after another (mhy_run per problem)"},
-        "batched_over_plain": value / plain,
+        "metric": METRIC, "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": r["dtype"], "data": "synthetic",
+        "config": {"workload": r["workload"], "problems": r["problems"], "cells": r["cells"],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "whole step (batched)", "achieved": B * r["value"] / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B * r["value"] / HBM_PEAK, "traffic": None},
+        "plain_form": r["plain_form"],
+        "batched_over_plain": r["batched_over_plain"],
     }
     print(json.dumps(out), flush=True)
 
@@ -195,12 +224,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c3")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--ensemble", type=int, default=512, help="problems of the c5 ensemble")
     ap.add_argument("--plain-steps", type=int, default=5, help="steps of the one-by-one comparison (c5)")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="leave out the ensemble comparison the default (c3) line carries")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
                     help="fp64 or mixed FP32/FP64 (default: mixed for c4 and c5, fp64 otherwise)")
     ap.add_argument("--report-steps", type=int, default=500,
@@ -235,8 +266,10 @@ def main():
         cfg.projection_order = args.projection_order
         desc = f"{desc}, remap order {args.projection_order}"
         traffic_key = f"{args.config}_o{args.projection_order}"
+    progress(f"{desc}: set-up")
     ctx = mahyco_amd.Context(cfg)
     cells = cfg.nx * cfg.ny
+    progress("warm-up and timed steps")
     ctx.run(args.warmup)
     ctx.synchronize()
     if args.kernel_timing == "inline":
@@ -315,6 +348,10 @@ def main():
     if not args.no_cpu_baseline:
         cfg.precision = 0
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    if args.config == "c3" and not args.no_ensemble:
+        ens_args = argparse.Namespace(**vars(args))
+        ens_args.precision = None  # the ensemble's own form: mixed, against plain FP64
+        out["ensemble"] = ensemble_rates(ens_args)
     print(json.dumps(out), flush=True)
 
 

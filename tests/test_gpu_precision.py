@@ -2,19 +2,24 @@
 fields and FP32 slope / limiter / PenteBorne arithmetic, FP64 volume
 fractions, masses, remap vectors and compensated flux accumulation.
 
-Its fields are compared with the FP64 form on the same input, which the
-parity tests pin bitwise to the CPU oracle.  Tolerances (relative L1 of
-density and energy <= 1e-5, the precision-form target) are written in each
-test; the total mass must stay conserved (MASS_TOL) and the volume fractions
-in [0, 1] with unit sum.  Batched mixed problems must equal the same problems
+Its fields are compared with the CPU oracle (FP64) stepped beside it on the
+same input, at the option sets' small sizes; the three-material layout at
+its full size (3072x1024, 500 steps), where the oracle would take too long,
+is compared with the FP64 form of this library, which the parity tests pin
+bitwise to the oracle at that size.  Tolerances (relative L1 of density and
+energy <= 1e-5, the precision-form target) are written in each test; the
+total mass must stay conserved (MASS_TOL) and the volume fractions in
+[0, 1] with unit sum.  Batched mixed problems must equal the same problems
 stepped alone bitwise."""
+import math
 import os
 
 import numpy as np
 import pytest
 
 import mahyco_amd
-from mahyco_amd import configs, precision
+from mahyco_amd import abi, configs, precision
+from oracle import refcpu
 
 from gpu_helpers import log
 from test_gpu_parity import extension_cases
@@ -32,12 +37,27 @@ MASS_TOL = 1e-10
 
 
 def _pair(cfg):
+    """The FP64 form of this library and the mixed form on the same input."""
     cfg.precision = 0
     ref = mahyco_amd.Context(cfg)
     cfg.precision = 1
     mix = mahyco_amd.Context(cfg)
     cfg.precision = 0
     return ref, mix
+
+
+def _oracle_pair(cfg):
+    """The CPU oracle (FP64) and the mixed form on the same input."""
+    cfg.precision = 0
+    orc = refcpu.Oracle(cfg)
+    cfg.precision = 1
+    mix = mahyco_amd.Context(cfg)
+    cfg.precision = 0
+    return orc, mix
+
+
+def _oracle_mass(orc):
+    return math.fsum(orc.get("cell_mass"))
 
 
 def _check(ref, mix, cfg, what, mass0):
@@ -54,20 +74,40 @@ def _check(ref, mix, cfg, what, mass0):
     assert np.allclose(fs.sum(axis=0), 1.0, rtol=0, atol=1e-12), f"{what}: volume fractions do not sum to 1"
 
 
-def test_mixed_triple_layout_long_run():
+def test_mixed_triple_layout_long_run_against_oracle():
     """The three-material layout (192x64, jittered densities) over 500
-    steps, well past the initial dt ramp."""
+    steps, well past the initial dt ramp, against the CPU oracle."""
     cfg = configs.triple_layout(192, 64)
+    orc, mix = _oracle_pair(cfg)
+    try:
+        mass0 = mix.conservation()[0]
+        log("[mixed] triple layout 192x64, 500 steps, against the oracle")
+        for chunk in (100, 400):
+            assert orc.step(chunk) == abi.MHY_OK
+            mix.run(chunk)
+        ta, tb = orc.time_state(), mix.time_state()
+        assert abs(ta.t_n - tb.t_n) <= 1e-5 * ta.t_n
+        _check(orc, mix, cfg, "triple layout 500 steps", mass0)
+    finally:
+        orc.close()
+        mix.close()
+
+
+def test_mixed_triple_layout_full_size():
+    """The three-material layout at its full size (3072x1024) over its 500
+    steps, against the FP64 form (bitwise the oracle at this size after the
+    steps the full-size parity test runs)."""
+    cfg = configs.triple_layout()
     ref, mix = _pair(cfg)
     try:
         mass0 = ref.conservation()[0]
-        log("[mixed] triple layout 192x64, 500 steps")
+        log("[mixed] triple layout 3072x1024, 500 steps, against the FP64 form")
         for chunk in (100, 400):
             ref.run(chunk)
             mix.run(chunk)
         ta, tb = ref.time_state(), mix.time_state()
         assert abs(ta.t_n - tb.t_n) <= 1e-5 * ta.t_n
-        _check(ref, mix, cfg, "triple layout 500 steps", mass0)
+        _check(ref, mix, cfg, "triple layout 3072x1024, 500 steps", mass0)
     finally:
         ref.close()
         mix.close()
@@ -76,17 +116,18 @@ def test_mixed_triple_layout_long_run():
 @pytest.mark.parametrize("name", sorted(extension_cases()))
 def test_mixed_options(name):
     """Every option set of the parity suite (PenteBorne variants, A1/A2/PB
-    dual mass fluxes, mixte flags, limiters, boundary fluxes) over 200 steps."""
+    dual mass fluxes, mixte flags, limiters, boundary fluxes) over 200
+    steps, against the CPU oracle."""
     cfg = extension_cases()[name]
-    ref, mix = _pair(cfg)
+    orc, mix = _oracle_pair(cfg)
     try:
-        log(f"[mixed] {name} {cfg.nx}x{cfg.ny} m={cfg.nbmat}, 200 steps")
-        ref.run(200)
+        log(f"[mixed] {name} {cfg.nx}x{cfg.ny} m={cfg.nbmat}, 200 steps, against the oracle")
+        assert orc.step(200) == abi.MHY_OK
         mix.run(200)
-        # the FP64 run's total mass (inflow boundaries change the mass)
-        _check(ref, mix, cfg, name, ref.conservation()[0])
+        # the oracle's total mass (inflow boundaries change the mass)
+        _check(orc, mix, cfg, name, _oracle_mass(orc))
     finally:
-        ref.close()
+        orc.close()
         mix.close()
 
 
