@@ -241,6 +241,17 @@ int mhy_set_field(mhy_ctx* ctx, int field, const double* in, long long n);
 /* sums[0] = total mass, sums[1] = total internal energy (sum of M*e),
  * sums[2..4] = mass of material k; FP64 compensated device reductions. */
 int mhy_conservation(mhy_ctx* ctx, double* sums);
+/* The same sums, allocating nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (tight serving loops, code
+ * around HIP graph capture, harnesses that forbid hipMalloc).  The block
+ * partials live in a static device array of 5120 doubles (40 KiB) per GPU,
+ * held until the process ends; every mesh fits it (at most 256 blocks x 5
+ * quantities x 4 partials), and a larger request would return MHY_ERR_ARG.
+ * Calls on the same GPU share that array: calls that overlap in time (several
+ * host threads or streams) race.  Results are bitwise those of
+ * mhy_conservation. */
+int mhy_conservation_noalloc(mhy_ctx* ctx, double* sums);
+
 
 /* Ensembles (no counterpart in the reference, which runs one problem per
  * process): n independent VNR Lagrange + remap problems of any mesh sizes,

@@ -20,7 +20,8 @@ EXPORTS = [
     "mhy_run", "mhy_simulate", "mhy_output_open", "mhy_output_write", "mhy_compute_dt", "mhy_lagrange",
     "mhy_prepare_remap", "mhy_remap_sweep", "mhy_remap_stage",
     "mhy_remap_final", "mhy_end_iteration", "mhy_synchronize", "mhy_get_time_state", "mhy_get_mesh_size",
-    "mhy_get_field", "mhy_set_field", "mhy_conservation", "mhy_kernel_count", "mhy_kernel_name",
+    "mhy_get_field", "mhy_set_field", "mhy_conservation", "mhy_conservation_noalloc", "mhy_kernel_count",
+    "mhy_kernel_name",
     "mhy_kernel_bytes", "mhy_enable_kernel_timing", "mhy_set_kernel_timing_mask", "mhy_kernel_time",
     "mhy_batch_create", "mhy_batch_destroy", "mhy_batch_last_error", "mhy_batch_run", "mhy_batch_context",
     "mhy_batch_cells",
@@ -63,6 +64,7 @@ def library():
         L.mhy_get_field.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_longlong]
         L.mhy_set_field.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_longlong]
         L.mhy_conservation.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+        L.mhy_conservation_noalloc.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.mhy_kernel_count.argtypes = [P]
         L.mhy_kernel_name.argtypes = [P, ctypes.c_int]
         L.mhy_kernel_name.restype = ctypes.c_char_p

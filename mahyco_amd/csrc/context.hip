@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,39 @@ const char* kKernelNames[K_COUNT] = {"dt_reduce", "lagrange_node", "lagrange_cel
 
 constexpr int kBlock = 256;
 constexpr int kReduceBlocks = 1024;
+
+// Block partial sums of the conservation reduction: at most
+// kConsBlocks blocks x (2 + MHY_NBMATMAX) quantities x (sum, compensation)
+// x 2 passes.
+constexpr int kConsBlocks = 256;
+constexpr int kConsScratch = kConsBlocks * (2 + MHY_NBMATMAX) * 4;  // doubles (40 KiB)
+
+}  // namespace
+
+// Scratch of mhy_conservation_noalloc: one copy per GPU, held until the
+// process ends.  Host code never passes this name to a kernel: the address
+// of each device's copy comes from hipGetSymbolAddress at context set-up
+// (scratchAddress) and is what the kernels receive.
+static __device__ __align__(256) double scratch[kConsScratch];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchLock;
+double* g_scratchAddr[kMaxDevices] = {nullptr};
+
+// Looks up this device's copy of the scratch array once per device (the
+// device must be current); returns nullptr on failure.
+double* scratchAddress(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratchLock);
+  if (!g_scratchAddr[device]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratchAddr[device] = static_cast<double*>(ptr);
+  }
+  return g_scratchAddr[device];
+}
 
 }  // namespace
 
@@ -71,6 +105,8 @@ struct mhy_ctx {
   bool useGraph = true;
   hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
   int status = MHY_OK;
+  // this device's copy of the scratch array (mhy_conservation_noalloc)
+  double* scratch = nullptr;
   long ncell = 0, nnode = 0, nface = 0;
   std::string err;
   std::vector<void*> allocs;
@@ -530,6 +566,8 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
 
   hipError_t e = hipSuccess;
   if ((e = hipGetDevice(&x->device)) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, hipGetErrorString(e)));
+  if (!(x->scratch = scratchAddress(x->device)))
+    return bail(fail(x, MHY_ERR_DEVICE, "hipGetSymbolAddress of the scratch array failed"));
   if ((e = hipStreamCreate(&x->stream)) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, hipGetErrorString(e)));
   Fields& f = x->f;
   std::memset(&f, 0, sizeof(f));
@@ -837,21 +875,14 @@ int mhy_set_field(mhy_ctx* x, int field, const double* in, long long n) {
   return MHY_OK;
 }
 
-int mhy_conservation(mhy_ctx* x, double* sums) {
-  if (!x || !sums) return MHY_ERR_ARG;
-  const int m = x->o.nbmat, nq = 2 + m;
-  const unsigned nb = (unsigned)std::min<long>(256, (x->ncell + kBlock - 1) / kBlock);
-  double* d = nullptr;
-  HIPCHK(x, hipMalloc(&d, sizeof(double) * nb * nq * 4));
-  KP kp = makeKP(x);
-  launchR(x, k_conservation<double>, k_conservation<float>, dim3(nb), dim3(kBlock), kp, d);
-  std::vector<double> h(nb * nq * 4);
-  hipError_t e1 = hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, x->stream);
-  hipError_t e2 = hipStreamSynchronize(x->stream);
-  (void)hipFree(d);
-  if (e1 != hipSuccess || e2 != hipSuccess) return fail(x, MHY_ERR_DEVICE, "conservation");
+}  // extern "C"
+
+namespace {
+
+// Neumaier sum of the block partials (sum and compensation of both passes)
+// of each quantity: the final step of both conservation entry points.
+void consFinish(const double* h, unsigned nb, int nq, double* sums) {
   for (int q = 0; q < nq; ++q) {
-    // Neumaier sum of the block sums and compensations
     double t = 0.0, cc = 0.0;
     for (unsigned bl = 0; bl < nb; ++bl)
       for (int part = 0; part < 4; ++part) {
@@ -863,6 +894,54 @@ int mhy_conservation(mhy_ctx* x, double* sums) {
       }
     sums[q] = t + cc;
   }
+}
+
+unsigned consBlocks(const mhy_ctx* x) {
+  return (unsigned)std::min<long>(kConsBlocks, (x->ncell + kBlock - 1) / kBlock);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhy_conservation(mhy_ctx* x, double* sums) {
+  if (!x || !sums) return MHY_ERR_ARG;
+  const int m = x->o.nbmat, nq = 2 + m;
+  const unsigned nb = consBlocks(x);
+  double* d = nullptr;
+  HIPCHK(x, hipMalloc(&d, sizeof(double) * nb * nq * 4));
+  KP kp = makeKP(x);
+  launchR(x, k_conservation<double>, k_conservation<float>, dim3(nb), dim3(kBlock), kp, d);
+  std::vector<double> h(nb * nq * 4);
+  hipError_t e1 = hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, x->stream);
+  hipError_t e2 = hipStreamSynchronize(x->stream);
+  (void)hipFree(d);
+  if (e1 != hipSuccess || e2 != hipSuccess) return fail(x, MHY_ERR_DEVICE, "conservation");
+  consFinish(h.data(), nb, nq, sums);
+  return MHY_OK;
+}
+
+// The same sums with the block partials in this device's copy of the
+// scratch array instead of a buffer allocated per call.  Every slot the
+// finish reads is written by the kernel of the same call, so nothing is
+// cleared.  Calls on the same GPU share the array: calls that overlap (from
+// several host threads or streams) race.
+int mhy_conservation_noalloc(mhy_ctx* x, double* sums) {
+  if (!x || !sums) return MHY_ERR_ARG;
+  const int m = x->o.nbmat, nq = 2 + m;
+  const unsigned nb = consBlocks(x);
+  const size_t need = (size_t)nb * nq * 4;
+  if (need > (size_t)kConsScratch)
+    return fail(x, MHY_ERR_ARG, "conservation partials (" + std::to_string(need) + " doubles) exceed the scratch "
+                                "array (" + std::to_string(kConsScratch) + ")");
+  if (!x->scratch) return fail(x, MHY_ERR_DEVICE, "scratch array not set up on this device");
+  KP kp = makeKP(x);
+  launchR(x, k_conservation<double>, k_conservation<float>, dim3(nb), dim3(kBlock), kp, x->scratch);
+  double h[kConsScratch];
+  hipError_t e1 = hipMemcpyAsync(h, x->scratch, sizeof(double) * need, hipMemcpyDeviceToHost, x->stream);
+  hipError_t e2 = hipStreamSynchronize(x->stream);
+  if (e1 != hipSuccess || e2 != hipSuccess) return fail(x, MHY_ERR_DEVICE, "conservation");
+  consFinish(h, nb, nq, sums);
   return MHY_OK;
 }
 
