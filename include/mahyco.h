@@ -188,6 +188,16 @@ typedef struct mhy_ctx mhy_ctx;
 int mhy_read_donnees(const char* path, mhy_config* cfg);
 /* Defaults of includes/ headers (the structs main.cc:17-25 news). */
 void mhy_config_default(mhy_config* cfg);
+/* JSON input: the keys of donnees.txt (LectureDonnees.cc:12-226) by name,
+ * the keys the positional format has no place for (per-material gammap,
+ * the MHY_CASE_* extension initialisers by name or id, param, seed,
+ * precision "fp64" / "mixed", abort_on_energy), an optional base file
+ * "donnees" (path relative to the JSON file) and an optional "problems"
+ * array, one configuration per entry, each starting from the keys around
+ * it (an ensemble for mhy_batch_create).  Writes *count configurations into
+ * cfgs; capacity 0 (cfgs may be NULL) only returns the count.  Unknown keys
+ * or malformed JSON: MHY_ERR_IO; capacity too small: MHY_ERR_ARG. */
+int mhy_read_json(const char* path, mhy_config* cfgs, int capacity, int* count);
 
 /* Vnr::Vnr + simulate() prologue (VnrRemap.cc:330-341): builds the mesh,
  * initial state (Init.cc), uploads it and runs setUpTimeLoopN. */
@@ -197,9 +207,18 @@ const char* mhy_last_error(const mhy_ctx* ctx);
 
 /* One iteration of executeTimeLoopN (VnrRemap.cc:130-222) including the
  * n/n+1 swap; no output.  Enqueued on the context stream; scalars stay on
- * the device unless sync != 0. */
+ * the device unless sync != 0.  Neither mhy_step nor mhy_run applies the
+ * loop's stopping test (final_time, max_time_iterations): the caller
+ * decides how many steps to run; after mhy_simulate has ended the loop both
+ * return MHY_ERR_FINISHED. */
 int mhy_step(mhy_ctx* ctx, int sync);
-/* nsteps iterations without host synchronisation in between. */
+/* nsteps iterations without host synchronisation in between.  The error
+ * status (|e| > 5 abort, NaN after the final remap) is read once, after the
+ * nsteps: when a step fails, the later steps of the call still run, so the
+ * fields, time and iteration count returned are those after nsteps steps,
+ * not those of the failing step (the reference stops at that step).  A
+ * caller that needs the state of the failing step steps with
+ * mhy_step(ctx, 1) or uses mhy_simulate, which test the status every step. */
 int mhy_run(mhy_ctx* ctx, int nsteps);
 /* Whole reference time loop (VnrRemap.cc:127-253) with VTP/PVD output in
  * output_dir (NULL or "none": no output) -- the drop-in for Vnr::simulate. */
@@ -251,7 +270,6 @@ int mhy_conservation(mhy_ctx* ctx, double* sums);
  * host threads or streams) race.  Results are bitwise those of
  * mhy_conservation. */
 int mhy_conservation_noalloc(mhy_ctx* ctx, double* sums);
-
 
 /* Ensembles (no counterpart in the reference, which runs one problem per
  * process): n independent VNR Lagrange + remap problems of any mesh sizes,

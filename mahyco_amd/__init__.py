@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmahyco_hip.so")
 
 # every entry point declared in include/mahyco.h
 EXPORTS = [
-    "mhy_read_donnees", "mhy_config_default", "mhy_create", "mhy_destroy", "mhy_last_error", "mhy_step",
+    "mhy_read_donnees", "mhy_read_json", "mhy_config_default", "mhy_create", "mhy_destroy", "mhy_last_error", "mhy_step",
     "mhy_run", "mhy_simulate", "mhy_output_open", "mhy_output_write", "mhy_compute_dt", "mhy_lagrange",
     "mhy_prepare_remap", "mhy_remap_sweep", "mhy_remap_stage",
     "mhy_remap_final", "mhy_end_iteration", "mhy_synchronize", "mhy_get_time_state", "mhy_get_mesh_size",
@@ -43,6 +43,8 @@ def library():
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.c_void_p
         L.mhy_read_donnees.argtypes = [ctypes.c_char_p, ctypes.POINTER(MhyConfig)]
+        L.mhy_read_json.argtypes = [ctypes.c_char_p, ctypes.POINTER(MhyConfig), ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int)]
         L.mhy_config_default.argtypes = [ctypes.POINTER(MhyConfig)]
         L.mhy_config_default.restype = None
         L.mhy_create.argtypes = [ctypes.POINTER(MhyConfig), ctypes.POINTER(P)]
@@ -91,6 +93,20 @@ def read_donnees(path):
     if st != MHY_OK:
         raise MahycoError(f"cannot read {path}: status {st}")
     return cfg
+
+
+def read_json(path):
+    """The JSON input (mhy_read_json): a list of MhyConfig, one per problem."""
+    n = ctypes.c_int(0)
+    L = library()
+    st = L.mhy_read_json(path.encode(), None, 0, ctypes.byref(n))
+    if st != MHY_OK:
+        raise MahycoError(f"cannot read {path}: status {st}")
+    arr = (MhyConfig * n.value)()
+    st = L.mhy_read_json(path.encode(), arr, n.value, ctypes.byref(n))
+    if st != MHY_OK:
+        raise MahycoError(f"cannot read {path}: status {st}")
+    return list(arr)
 
 
 class Context:

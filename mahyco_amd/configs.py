@@ -130,3 +130,49 @@ def ensemble(n=512, seed=20261204, side_min=64, side_max=1024):
         x0 = rng.uniform(0.3, 0.7)
         out.append(riemann(side, float(x0), float(rho_l), float(p_l), float(rho_r), float(p_r), float(g_l), float(g_r)))
     return out
+
+
+# ------------------------------------------------------------ JSON input
+_CASE_NAME = {v: k for k, v in CASE.items()}
+_LIM_NAME = {v: k for k, v in LIM.items()}
+
+
+def as_json_dict(c):
+    """Every key of one configuration in the JSON input's terms
+    (mhy_read_json): names for the test case and limiters, ids elsewhere."""
+    d = {}
+    for name, _ in MhyConfig._fields_:
+        v = getattr(c, name)
+        if name in ("eos", "gammap", "param"):
+            v = list(v)
+        elif name == "test_case":
+            v = _CASE_NAME.get(v, v)
+        elif name in ("limiter", "limiter_pure"):
+            v = _LIM_NAME.get(v, v)
+        elif name == "precision":
+            v = "mixed" if v == 1 else "fp64"
+        d[name] = v
+    return d
+
+
+def to_json(cfgs):
+    """The JSON input of one configuration or of an ensemble: keys shared by
+    every problem at the top level, the others in "problems"."""
+    import json
+    dicts = [as_json_dict(c) for c in cfgs]
+    common = {k: v for k, v in dicts[0].items() if all(d[k] == v for d in dicts)}
+    doc = dict(common)
+    if len(dicts) > 1:
+        doc["problems"] = [{k: v for k, v in d.items() if k not in common} for d in dicts]
+    return json.dumps(doc, indent=1)
+
+
+WORKLOADS = {"c2": lambda: [riemann_perturbed()], "c3": lambda: [point_explosion()],
+             "c4": lambda: [triple_layout()], "c5": ensemble}
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) != 2 or sys.argv[1] not in WORKLOADS:
+        raise SystemExit("usage: python -m mahyco_amd.configs c2|c3|c4|c5 > workload.json")
+    print(to_json(WORKLOADS[sys.argv[1]]()))
