@@ -269,6 +269,14 @@ def main():
     progress(f"{desc}: set-up")
     ctx = mahyco_amd.Context(cfg)
     cells = cfg.nx * cfg.ny
+    # the launch-shape tuner's steps (meshes of at least 2^20 cells) are part
+    # of the set-up: they time the interchangeable y kernels before any
+    # graph is captured, and come before the warm-up
+    tuning_steps = 0
+    if cells >= (1 << 20):
+        while not ctx.tuned_kernels() and tuning_steps < 32:
+            ctx.run(1)
+            tuning_steps += 1
     progress("warm-up and timed steps")
     ctx.run(args.warmup)
     ctx.synchronize()
@@ -297,6 +305,7 @@ def main():
         tot = {k: v["avg_ms"] * v["launches"] for k, v in stats.items()}
         dom = max(tot, key=tot.get)
     ctx.enable_timing(False)
+    tuned = ctx.tuned_kernels()
     ctx.close()
     dk = stats[dom]
     achieved = dk["bytes"] / (dk["avg_ms"] * 1e-3) if dk["avg_ms"] > 0 else 0.0
@@ -328,6 +337,8 @@ def main():
                           "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": B * value / HBM_PEAK},
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "GBps_alg": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)
                         if v["avg_ms"] > 0 else 0} for k, v in stats.items() if v["launches"]},
+        "tuned_kernels": tuned or None,
+        "tuning_steps": tuning_steps,
         "kernel_timing": ("HIP events on the context stream: the dominant kernel over the timed steps, the others "
                           "over an untimed pass just before" if args.kernel_timing == "inline" else
                           "HIP events on the context stream over a few steps after the timed ones"),

@@ -21,7 +21,7 @@ EXPORTS = [
     "mhy_prepare_remap", "mhy_remap_sweep", "mhy_remap_stage",
     "mhy_remap_final", "mhy_end_iteration", "mhy_synchronize", "mhy_get_time_state", "mhy_get_mesh_size",
     "mhy_get_field", "mhy_set_field", "mhy_conservation", "mhy_conservation_noalloc", "mhy_kernel_count",
-    "mhy_kernel_name",
+    "mhy_kernel_name", "mhy_tuned_kernels",
     "mhy_kernel_bytes", "mhy_enable_kernel_timing", "mhy_set_kernel_timing_mask", "mhy_kernel_time",
     "mhy_batch_create", "mhy_batch_destroy", "mhy_batch_last_error", "mhy_batch_run", "mhy_batch_context",
     "mhy_batch_cells",
@@ -68,6 +68,8 @@ def library():
         L.mhy_conservation.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.mhy_conservation_noalloc.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.mhy_kernel_count.argtypes = [P]
+        L.mhy_tuned_kernels.argtypes = [P]
+        L.mhy_tuned_kernels.restype = ctypes.c_char_p
         L.mhy_kernel_name.argtypes = [P, ctypes.c_int]
         L.mhy_kernel_name.restype = ctypes.c_char_p
         L.mhy_kernel_bytes.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -180,6 +182,10 @@ class Context:
         out = (ctypes.c_double * (2 + 3))()
         self._check(library().mhy_conservation(self.h, out), "mhy_conservation")
         return list(out)[: 2 + self.cfg.nbmat]
+
+    def tuned_kernels(self):
+        """The y-direction kernel variants the launch-shape tuner kept."""
+        return library().mhy_tuned_kernels(self.h).decode()
 
     def enable_timing(self, on=True):
         self._check(library().mhy_enable_kernel_timing(self.h, int(on)), "timing")

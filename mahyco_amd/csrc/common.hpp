@@ -162,6 +162,8 @@ struct KPT {
   int sweep;     // 1 or 2
   int xdir;      // 1: sweep along x, 0: along y
   int fuseFinal; // sweep 2 finishes each cell (final remap) instead of storing U2
+  int geomLoad;  // the sweeps read the Lagrange volume and centroid (V_L, Xc_L, Yc_L)
+                 // instead of forming them again from the node velocities
   long ncell, nnode;
 };
 

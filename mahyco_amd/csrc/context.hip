@@ -22,9 +22,12 @@ using namespace mhy;
 
 namespace {
 
-enum KernelId { K_DT = 0, K_NODE, K_CELL, K_SWEEP_X, K_SWEEP_Y, K_DUAL_X, K_DUAL_Y, K_FINAL, K_COUNT };
-const char* kKernelNames[K_COUNT] = {"dt_reduce", "lagrange_node", "lagrange_cell", "sweep_x",
-                                     "sweep_y",   "dual_x",        "dual_y",        "remap_final"};
+enum KernelId {
+  K_DT = 0, K_NODE, K_CELL, K_SWEEP_X, K_SWEEP_Y, K_DUAL_X, K_DUAL_Y, K_FINAL, K_LAGSWEEP_X, K_LAGSWEEP_Y, K_COUNT
+};
+const char* kKernelNames[K_COUNT] = {"dt_reduce", "lagrange_node", "lagrange_cell",    "sweep_x",
+                                     "sweep_y",   "dual_x",        "dual_y",           "remap_final",
+                                     "lagrange_sweep_x", "lagrange_sweep_y"};
 
 constexpr int kBlock = 256;
 constexpr int kReduceBlocks = 1024;
@@ -85,6 +88,19 @@ struct mhy_ctx {
   // MAHYCO_FUSE_FINAL=0 keeps it a separate kernel); finalDone records that
   // the current iteration's final remap is already done
   bool fuseFinal = true, finalDone = false;
+  // sweep 1 runs the cell part of the Lagrange phase (wave sweeps only;
+  // environment MAHYCO_FUSE_LAGRANGE=0 keeps it a separate kernel)
+  bool fuseLag = true;
+  // ... also on steps whose first sweep runs along y (the tuner may turn it
+  // off there: the y strips recompute a third of their cells as halo)
+  bool fuseLagY = true;
+  // with the fused first sweep: that sweep stores the Lagrange volume and
+  // centroid for the second (environment MAHYCO_SWEEP_GEOM=load), or the
+  // second sweep forms them again from the node velocities (=recompute)
+  bool geomStored = true;
+  // node update on the Euler grid as an interior kernel and a boundary
+  // kernel (MAHYCO_SPLIT_NODE=0: one kernel for all nodes)
+  bool splitNode = true;
   // nodal remap along y: the wave kernel (default) or the per-node stencil
   // kernel (environment MAHYCO_DUAL_Y=node)
   bool dualWaveY = true;
@@ -93,6 +109,9 @@ struct mhy_ctx {
   // faster (fewer waves leave the chip underfilled); MAHYCO_DUAL_Y=walk /
   // wave / node forces a kernel
   bool dualWalkY = false;
+  // nodal remap along y on LDS tiles of 16 node columns x 64 rows
+  // (MAHYCO_DUAL_Y=tile)
+  bool dualTileY = false;
   // cell sweeps along y: the column-walking kernel or the wave-strip
   // kernel.  The walker keeps more state in registers (fewer waves per SIMD)
   // but has less halo work; it is chosen for the mixed form with one
@@ -121,6 +140,26 @@ struct mhy_ctx {
   std::vector<hipEvent_t> eventPool;
   double ms[K_COUNT] = {0};
   int launches[K_COUNT] = {0};
+  // Launch-shape tuner.  The y-direction kernels come in interchangeable
+  // variants with different tilings (results bitwise the same): the nodal
+  // remap as 16x64-node LDS tiles, 4x16-node wave strips or 64-column
+  // walking bands; the cell sweep as 4x16-cell wave strips or 64-column
+  // walking bands; and, on steps whose first sweep runs along y, the
+  // Lagrange cell update fused into that sweep or run as its own kernel.  On
+  // a large problem the first steps time whole steps, one site after the
+  // other (two steps per candidate, so that both sweep orders are seen; the
+  // other sites keep their current choice), and keep the fastest; the
+  // choice is fixed before any graph is captured.
+  struct Tuner {
+    bool on = false;
+    int step = 0, steps = 0;
+    int ncand[3] = {0, 0, 0};  // site 0: nodal remap along y, 1: cell sweep along y, 2: y-first Lagrange
+    int cand[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    int best[3] = {0, 0, 0};  // index into cand of the current choice
+    double ms[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    hipEvent_t ev[2] = {nullptr, nullptr};  // around the current step
+    std::string choice;  // the kept variants, for reports
+  } tune;
   // output
   OutputWriter* writer = nullptr;
 };
@@ -187,9 +226,92 @@ KP makeKP(const mhy_ctx* x) {
   kp.sweep = 1;
   kp.xdir = 1;
   kp.fuseFinal = 0;
+  // the stored Lagrange geometry exists unless the fused first sweep keeps
+  // it in registers only
+  kp.geomLoad = (!x->fuseLag || x->geomStored) ? 1 : 0;
   kp.ncell = x->ncell;
   kp.nnode = x->nnode;
   return kp;
+}
+
+// candidate ids of the tuner's sites
+enum { TD_TILE = 0, TD_WAVE = 1, TD_WALK = 2 };  // nodal remap along y
+enum { TS_WAVE = 0, TS_WALK = 1 };               // cell sweep along y
+enum { TL_FUSED = 0, TL_SEPARATE = 1 };          // Lagrange cell update of y-first steps
+const char* kDualNames[3] = {"tile", "wave", "walk"};
+const char* kSweepNames[2] = {"wave", "walk"};
+const char* kLagNames[2] = {"fused", "separate"};
+
+void setDualY(mhy_ctx* x, int c) {
+  x->dualTileY = (c == TD_TILE);
+  x->dualWalkY = (c == TD_WALK);
+  x->dualWaveY = true;
+}
+
+// Applies candidate index ci of a site.
+void tuneApply(mhy_ctx* x, int site, int ci) {
+  const int id = x->tune.cand[site][ci];
+  if (site == 0) setDualY(x, id);
+  else if (site == 1) x->walkY = (id == TS_WALK);
+  else x->fuseLagY = (id == TL_FUSED);
+}
+
+// The site and candidate index of tuning step k (sites one after the other,
+// two steps per candidate); false past the schedule.
+bool tuneSlot(const mhy_ctx* x, int k, int& site, int& ci) {
+  const auto& t = x->tune;
+  for (site = 0; site < 3; ++site) {
+    if (k < 2 * t.ncand[site]) {
+      ci = k / 2;
+      return true;
+    }
+    k -= 2 * t.ncand[site];
+  }
+  return false;
+}
+
+// Selects the candidate of the current tuning step and starts its timing
+// (called at the start of every step while tuning).
+void tuneSelect(mhy_ctx* x) {
+  auto& t = x->tune;
+  if (!t.on) return;
+  int site, ci;
+  if (!tuneSlot(x, t.step, site, ci)) return;
+  tuneApply(x, site, ci);
+  (void)hipEventRecord(t.ev[0], x->stream);
+}
+
+// Ends a tuning step: adds its time to its candidate; after the last step
+// of a site, keeps the fastest candidate of that site.
+void tuneStepDone(mhy_ctx* x) {
+  auto& t = x->tune;
+  if (!t.on) return;
+  int site, ci;
+  if (!tuneSlot(x, t.step, site, ci)) return;
+  (void)hipEventRecord(t.ev[1], x->stream);
+  float ms = 0.f;
+  if (hipEventSynchronize(t.ev[1]) == hipSuccess && hipEventElapsedTime(&ms, t.ev[0], t.ev[1]) == hipSuccess)
+    t.ms[site][ci] += ms;
+  ++t.step;
+  int nsite, nci;
+  const bool more = tuneSlot(x, t.step, nsite, nci);
+  if (!more || nsite != site) {  // the site is done
+    int b = 0;
+    for (int c = 1; c < t.ncand[site]; ++c)
+      if (t.ms[site][c] < t.ms[site][b]) b = c;
+    t.best[site] = b;
+    tuneApply(x, site, b);
+  }
+  if (!more) {
+    t.on = false;
+    t.choice.clear();
+    auto add = [&](const char* k, const char* v) {
+      t.choice += std::string(t.choice.empty() ? "" : " ") + k + "=" + v;
+    };
+    if (t.ncand[0]) add("dual_y", kDualNames[t.cand[0][t.best[0]]]);
+    if (t.ncand[1]) add("sweep_y", kSweepNames[t.cand[1][t.best[1]]]);
+    if (t.ncand[2]) add("lagrange_y", kLagNames[t.cand[2][t.best[2]]]);
+  }
 }
 
 hipEvent_t takeEvent(mhy_ctx* x) {
@@ -250,6 +372,7 @@ int checkLaunch(mhy_ctx* x, const char* what) {
 }
 
 int phaseDt(mhy_ctx* x) {
+  tuneSelect(x);  // a step starts here
   KP kp = makeKP(x);
   if (x->o.mode == MODE_ADVECTION) {
     timed(x, K_DT, [&] { launchR(x, k_dt_hold<double>, k_dt_hold<float>, dim3(1), dim3(64), kp); });
@@ -266,15 +389,30 @@ int phaseDt(mhy_ctx* x) {
 using KernelFn = void (*)(KP);
 using KernelFnF = void (*)(KPT<float>);
 
+// the Lagrange cell update of this step runs inside the first sweep
+bool lagFused(const mhy_ctx* x) { return x->fuseLag && (x->x_then_y_n || x->fuseLagY); }
+
 int phaseLagrange(mhy_ctx* x) {
   KP kp = makeKP(x);
-  timed(x, K_NODE, [&] {
-    launchR(x, k_lagrange_node<double>, k_lagrange_node<float>, gridFor(x->nnode), dim3(kBlock), kp);
-  });
+  // (a grid's y dimension holds at most 65535 node rows)
+  if (x->o.mode == MODE_FULL && x->o.nx >= 2 && x->o.ny >= 2 && x->o.ny - 1 <= 65535 && x->splitNode) {
+    const dim3 gi((unsigned)((x->o.nx - 1 + kBlock - 1) / kBlock), (unsigned)(x->o.ny - 1));
+    const long nedge = 2L * (x->o.nx + 1) + 2L * (x->o.ny - 1);
+    timed(x, K_NODE, [&] {
+      launchR(x, k_lagrange_node_inner<double>, k_lagrange_node_inner<float>, gi, dim3(kBlock), kp);
+      launchR(x, k_lagrange_node_edge<double>, k_lagrange_node_edge<float>, gridFor(nedge), dim3(kBlock), kp);
+    });
+  } else {
+    timed(x, K_NODE, [&] {
+      launchR(x, k_lagrange_node<double>, k_lagrange_node<float>, gridFor(x->nnode), dim3(kBlock), kp);
+    });
+  }
   static const KernelFn kc[3] = {k_lagrange_cell<double, 1>, k_lagrange_cell<double, 2>, k_lagrange_cell<double, 3>};
   static const KernelFnF kcf[3] = {k_lagrange_cell<float, 1>, k_lagrange_cell<float, 2>, k_lagrange_cell<float, 3>};
   const int mi = x->o.nbmat - 1;
-  timed(x, K_CELL, [&] { launchR(x, kc[mi], kcf[mi], gridFor(x->ncell), dim3(kBlock), kp); });
+  // with the fused first sweep the cell update runs in that sweep
+  if (!(lagFused(x) && x->o.mode != MODE_LAGRANGE))
+    timed(x, K_CELL, [&] { launchR(x, kc[mi], kcf[mi], gridFor(x->ncell), dim3(kBlock), kp); });
   return checkLaunch(x, "lagrange");
 }
 
@@ -282,8 +420,16 @@ int phaseLagrange(mhy_ctx* x) {
 // option: the wave-shuffle kernels (default) or the per-cell stencil kernels
 // (FP64 form only)
 template <class R>
-void (*sweepWaveKernel(bool xd, int m, bool pp))(KPT<R>) {
+void (*sweepWaveKernel(bool xd, int m, bool pp, bool lag = false))(KPT<R>) {
   using Fn = void (*)(KPT<R>);
+  static const Fn kl[2][3][2] = {
+      {{k_sweep_wave_lag<R, 1, false, false>, k_sweep_wave_lag<R, 1, true, false>},
+       {k_sweep_wave_lag<R, 2, false, false>, k_sweep_wave_lag<R, 2, true, false>},
+       {k_sweep_wave_lag<R, 3, false, false>, k_sweep_wave_lag<R, 3, true, false>}},
+      {{k_sweep_wave_lag<R, 1, false, true>, k_sweep_wave_lag<R, 1, true, true>},
+       {k_sweep_wave_lag<R, 2, false, true>, k_sweep_wave_lag<R, 2, true, true>},
+       {k_sweep_wave_lag<R, 3, false, true>, k_sweep_wave_lag<R, 3, true, true>}}};
+  if (lag) return kl[xd ? 1 : 0][m - 1][pp ? 1 : 0];
   static const Fn kw[2][3][2] = {
       {{k_sweep_wave<R, 1, false, false>, k_sweep_wave<R, 1, true, false>},
        {k_sweep_wave<R, 2, false, false>, k_sweep_wave<R, 2, true, false>},
@@ -324,7 +470,8 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
                       : o3 ? sweepO3Grid(xd, x->o.nx, x->o.ny)
                            : gridFor(x->ncell);
     const dim3 block(wave ? sweepWaveBlock(xd, x->mixed, x->o.pp == 1) : kBlock);
-    if (!xd && wave && x->walkY) {
+    const bool lagSweep = (sweep == 1 && lagFused(x));
+    if (!xd && wave && x->walkY && !lagSweep) {
       static const KernelFn kd[3][2] = {{k_sweep_walk<double, 1, false>, k_sweep_walk<double, 1, true>},
                                         {k_sweep_walk<double, 2, false>, k_sweep_walk<double, 2, true>},
                                         {k_sweep_walk<double, 3, false>, k_sweep_walk<double, 3, true>}};
@@ -344,6 +491,15 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
           {k_sweep_o3<float, 1, true>, k_sweep_o3<float, 2, true>, k_sweep_o3<float, 3, true>}};
       const int di = xd ? 1 : 0, mi = x->o.nbmat - 1;
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { launchR(x, kd[di][mi], kf[di][mi], grid, block, kp); });
+    } else if (lagSweep) {  // with the cell part of the Lagrange phase
+      const KernelId kid = xd ? K_LAGSWEEP_X : K_LAGSWEEP_Y;
+      if (x->mixed) {
+        const KernelFnF k = sweepWaveKernel<float>(xd, x->o.nbmat, x->o.pp == 1, true);
+        timed(x, kid, [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, retype<float>(kp)); });
+      } else {
+        const KernelFn k = sweepWaveKernel<double>(xd, x->o.nbmat, x->o.pp == 1, true);
+        timed(x, kid, [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, kp); });
+      }
     } else if (x->mixed) {
       const KernelFnF k = sweepWaveKernel<float>(xd, x->o.nbmat, x->o.pp == 1);
       timed(x, xd ? K_SWEEP_X : K_SWEEP_Y,
@@ -358,14 +514,20 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
   } else if (xd) {
     timed(x, K_DUAL_X,
           [&] { launchR(x, k_dual<double, true>, k_dual<float, true>, gridFor(x->nnode), dim3(kBlock), kp); });
+  } else if (x->dualTileY) {
+    timed(x, K_DUAL_Y, [&] {
+      launchR(x, k_dual_tile<double>, k_dual_tile<float>, dualTileGrid(x->o.nx, x->o.ny), dim3(kBlock), kp);
+    });
   } else if (x->dualWalkY) {
     timed(x, K_DUAL_Y, [&] {
       launchR(x, k_dual_walk<double>, k_dual_walk<float>, dualWalkGrid(x->o.nx, x->o.ny), dim3(kBlock), kp);
     });
   } else if (x->sweepWave && x->dualWaveY) {
     const dim3 grid = dualWaveGrid(false, x->o.nx, x->o.ny, x->mixed);
-    timed(x, K_DUAL_Y,
-          [&] { launchR(x, k_dual_wave<double, false>, k_dual_wave<float, false>, grid, dim3(dualWaveBlock(false, x->mixed)), kp); });
+    timed(x, K_DUAL_Y, [&] {
+      launchR(x, k_dual_wave<double, false>, k_dual_wave<float, false>, grid, dim3(dualWaveBlock(false, x->mixed)),
+              kp);
+    });
   } else {
     timed(x, K_DUAL_Y,
           [&] { launchR(x, k_dual<double, false>, k_dual<float, false>, gridFor(x->nnode), dim3(kBlock), kp); });
@@ -400,6 +562,7 @@ int readScalars(mhy_ctx* x, Scalars& s) {
 
 // Swap of the n / n+1 buffers (VnrRemap.cc:198-222) = parity flip.
 void endIteration(mhy_ctx* x) {
+  tuneStepDone(x);
   x->cur = 1 - x->cur;
   if (x->o.mode != MODE_LAGRANGE) x->x_then_y_n = !x->x_then_y_n;
 }
@@ -496,7 +659,9 @@ extern "C" {
 
 const char* mhy_last_error(const mhy_ctx* x) { return x ? x->err.c_str() : "null context"; }
 
-int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
+// batchMember: a problem of an ensemble, whose steps the batch launches
+// (with the stand-alone Lagrange cell kernel)
+static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember) {
   if (!cfg || !out) return MHY_ERR_ARG;
   *out = nullptr;
   mhy_ctx* x = new mhy_ctx();
@@ -521,15 +686,64 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   if (x->mixed) x->sweepWave = true;  // the per-cell stencil sweeps are FP64 only
   x->walkY = x->mixed && c.nbmat == 1 && c.pp != 1 && (long)sweepWalkGrid(c.nx, c.ny).x * 4 >= 4096;
   x->dualWalkY = x->mixed && (long)dualWalkGrid(c.nx, c.ny).x * 4 >= 8192;
+  // FP64: the LDS-tile kernel (4096^2 point explosion: 0.42 ms against 0.45 ms
+  // for the wave strip)
+  x->dualTileY = !x->mixed;
   if (const char* dy = std::getenv("MAHYCO_DUAL_Y")) {
     x->dualWaveY = (std::string(dy) != "node");
     x->dualWalkY = (std::string(dy) == "walk");
+    x->dualTileY = (std::string(dy) == "tile");
   }
   if (const char* wy = std::getenv("MAHYCO_SWEEP_Y")) {
     if (std::string(wy) == "walk") x->walkY = true;
     if (std::string(wy) == "wave") x->walkY = false;
   }
   if (const char* g = std::getenv("MAHYCO_GRAPH")) x->useGraph = (std::string(g) != "0");
+  // the fused first sweep needs the wave kernels for both sweep directions
+  // (the column walker and the third-order and per-cell kernels read the
+  // stored remap vector).  The PenteBorne sweeps are bound by their FP64
+  // arithmetic, not by memory: the Lagrange update runs better as its own
+  // streaming kernel there (2048^2 two-material case: 1.24 ms per step
+  // separate, 1.27 ms fused), so it is fused with the limited-slope sweeps only.
+  // (a column-walking second sweep reads the stored Lagrange geometry)
+  if (const char* sg = std::getenv("MAHYCO_SWEEP_GEOM")) x->geomStored = (std::string(sg) != "recompute");
+  x->fuseLag = x->sweepWave && c.projection_order != 3 && c.avec_projection == 1 && !batchMember && c.pp != 1 &&
+               (!x->walkY || x->geomStored);
+  if (const char* fl = std::getenv("MAHYCO_FUSE_LAGRANGE"))
+    if (std::string(fl) == "0") x->fuseLag = false;
+  if (const char* sn = std::getenv("MAHYCO_SPLIT_NODE")) x->splitNode = (std::string(sn) != "0");
+  // the tuner (MAHYCO_AUTOTUNE=0 disables it; a variant forced through the
+  // environment is not tuned): meshes of at least 2^20 cells, where the
+  // variants' speeds differ, with the wave sweeps; the first candidate of
+  // each site is the default choice
+  {
+    bool tuneOn = (long)c.nx * c.ny >= (1L << 20) && x->sweepWave && c.projection_order != 3 &&
+                  c.avec_projection == 1 && !batchMember;
+    if (const char* at = std::getenv("MAHYCO_AUTOTUNE")) tuneOn = tuneOn && (std::string(at) != "0");
+    auto& t = x->tune;
+    if (tuneOn && !std::getenv("MAHYCO_DUAL_Y")) {
+      t.cand[0][0] = x->dualTileY ? TD_TILE : (x->dualWalkY ? TD_WALK : TD_WAVE);
+      int n = 1;
+      for (int c2 : {TD_TILE, TD_WAVE, TD_WALK})
+        if (c2 != t.cand[0][0]) t.cand[0][n++] = c2;
+      t.ncand[0] = n;
+    }
+    // the walking sweep reads the stored Lagrange geometry
+    const bool geom = !x->fuseLag || x->geomStored;
+    if (tuneOn && geom && !std::getenv("MAHYCO_SWEEP_Y")) {
+      t.cand[1][0] = x->walkY ? TS_WALK : TS_WAVE;
+      t.cand[1][1] = x->walkY ? TS_WAVE : TS_WALK;
+      t.ncand[1] = 2;
+    }
+    if (tuneOn && x->fuseLag) {
+      t.cand[2][0] = TL_FUSED, t.cand[2][1] = TL_SEPARATE;
+      t.ncand[2] = 2;
+    }
+    t.steps = 2 * (t.ncand[0] + t.ncand[1] + t.ncand[2]);
+    t.on = t.steps > 0;
+    if (t.on && (hipEventCreate(&t.ev[0]) != hipSuccess || hipEventCreate(&t.ev[1]) != hipSuccess))
+      return bail(fail(x, MHY_ERR_DEVICE, "tuner events"));
+  }
   if (c.sans_lagrange && !c.avec_projection)
     return bail(fail(x, MHY_ERR_ARG, "sansLagrange needs the projection"));
   std::string err;
@@ -538,6 +752,9 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   if (c.avec_projection && !outerNormalsAxisExact(c, err)) return bail(fail(x, MHY_ERR_ARG, err));
 
   const HostInit& h = x->init;
+  // the kernels index cells and nodes with 32-bit arithmetic
+  if (c.nx < 1 || c.ny < 1 || (long)(c.nx + 1) * (c.ny + 1) >= (1L << 31))
+    return bail(fail(x, MHY_ERR_ARG, "mesh size out of range (at most 2^31 nodes)"));
   x->ncell = (long)c.nx * c.ny;
   x->nnode = (long)(c.nx + 1) * (c.ny + 1);
   const long C = x->ncell, N = x->nnode;
@@ -597,6 +814,9 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
     for (int k = 0; k < m; ++k) AR(f.rho_k0[k], C), AR(f.e_k0[k], C);
   }
   if (o.mode != MODE_LAGRANGE) {
+    // U_L, V_L and the centroid: the fused first sweep keeps U_L in
+    // registers, but a step whose first sweep runs along y may run the
+    // separate Lagrange kernel (the tuner's choice)
     for (int q = 0; q < o.nc; ++q) AD(f.UL[q], C), AD(f.U1[q], C), AD(f.U2[q], C);
     AD(f.VL, C), AD(f.XcL, C), AD(f.YcL, C);
     if (o.methode == 1) AD(f.rL, C);
@@ -674,6 +894,8 @@ int mhy_create(const mhy_config* cfg, mhy_ctx** out) {
   return MHY_OK;
 }
 
+int mhy_create(const mhy_config* cfg, mhy_ctx** out) { return createContext(cfg, out, false); }
+
 int mhy_destroy(mhy_ctx* x) {
   if (!x) return MHY_OK;
   if (x->stream) (void)hipStreamSynchronize(x->stream);
@@ -681,6 +903,8 @@ int mhy_destroy(mhy_ctx* x) {
   for (auto ev : x->eventPool) (void)hipEventDestroy(ev);
   for (auto& g : x->graph)
     if (g) (void)hipGraphExecDestroy(g);
+  for (auto ev : x->tune.ev)
+    if (ev) (void)hipEventDestroy(ev);
   for (void* p : x->allocs) (void)hipFree(p);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x->writer;
@@ -691,6 +915,7 @@ int mhy_destroy(mhy_ctx* x) {
 int mhy_step(mhy_ctx* x, int sync) {
   if (!x) return MHY_ERR_ARG;
   if (x->status != MHY_OK) return x->status;
+  if (x->finished) return fail(x, MHY_ERR_FINISHED, "time loop already ended");
   int st = runIteration(x);
   if (st != MHY_OK) return st;
   endIteration(x);
@@ -727,7 +952,8 @@ int captureStep(mhy_ctx* x) {
 // One step (without the n/n+1 swap): a replay of the captured step when
 // graphs is set, else eager launches.
 int stepOnce(mhy_ctx* x, bool graphs) {
-  if (!graphs) return runIteration(x);
+  // no graph is captured before the tuner has fixed the kernel variants
+  if (!graphs || x->tune.on) return runIteration(x);
   hipGraphExec_t& g = x->graph[2 * x->cur + (x->x_then_y_n ? 1 : 0)];
   if (!g) {
     const int st = captureStep(x);
@@ -741,6 +967,7 @@ int stepOnce(mhy_ctx* x, bool graphs) {
 int mhy_run(mhy_ctx* x, int nsteps) {
   if (!x) return MHY_ERR_ARG;
   if (x->status != MHY_OK) return x->status;
+  if (x->finished) return fail(x, MHY_ERR_FINISHED, "time loop already ended");
   // graphs need the launches of a step to depend only on the parity; kernel
   // events (timing) are recorded on the eager path
   const bool graphs = x->useGraph && !x->timing && nsteps >= 2;
@@ -945,6 +1172,8 @@ int mhy_conservation_noalloc(mhy_ctx* x, double* sums) {
   return MHY_OK;
 }
 
+const char* mhy_tuned_kernels(const mhy_ctx* x) { return x ? x->tune.choice.c_str() : ""; }
+
 int mhy_kernel_count(const mhy_ctx*) { return K_COUNT; }
 const char* mhy_kernel_name(const mhy_ctx*, int k) { return (k >= 0 && k < K_COUNT) ? kKernelNames[k] : ""; }
 
@@ -973,14 +1202,32 @@ int mhy_kernel_bytes(const mhy_ctx* x, int k, double* bytes) {
       break;
     case K_SWEEP_X:
     case K_SWEEP_Y: {
-      // U_in, VL or qL, Xc (+ flags) ; u ; writes U_out, face fluxes (or PB)
-      const double in = C * d * (nc + 2) + C * i4 * 2 * (pp && x->o.pp_mixte) + N * 2 * r;
+      // U_in (+ qL, flags) ; both node velocity components (the wave sweeps
+      // form V_L and the centroid from the node positions; the other kernels
+      // read them) ; writes U_out, face fluxes (or PB)
+      const double geo = x->sweepWave && x->o.order != 3 ? 0.0 : C * d * 2;
+      const double in = C * d * nc + geo + C * d * pp + C * i4 * 2 * (pp && x->o.pp_mixte) + N * 2 * r;
       const double fl = (meth == 2 ? C * m : F * m) * d;
       const double s1 = in + C * nc * d + fl;
       // sweep 2 with the final remap fused: no U2 store, the final outputs
-      // instead; X and Y alternate between sweep 1 and sweep 2
+      // instead; X and Y alternate between sweep 1 and sweep 2, unless
+      // sweep 1 is the fused Lagrange sweep (its own kernel id)
       const double s2 = x->fuseFinal ? in + finalWrites + fl : s1;
-      w = 0.5 * (s1 + s2);
+      // a sweep direction whose first sweeps are the fused Lagrange sweep
+      // (their own kernel id) runs only second sweeps under this id
+      const bool lagOwn = x->fuseLag && (k == K_SWEEP_X || x->fuseLagY);
+      w = lagOwn ? s2 : 0.5 * (s1 + s2);
+      break;
+    }
+    case K_LAGSWEEP_X:
+    case K_LAGSWEEP_Y: {
+      // the Lagrange cell update's inputs (u_n+1 ; f, M_k, vol (FP64), mu,
+      // rho_k, e_k, p_k, q_k + rho, divu, c) and outputs that outlive the
+      // step (divu, q_k ; rL, qL), then sweep 1's outputs: U_1, face fluxes
+      const double lagIn = N * 2 * r + C * ((2 * m + 1) * d + (5 * m + 3) * r);
+      const double lagOut = C * r * (1 + m) + C * d * ((meth == 1) + pp);
+      const double fl = (meth == 2 ? C * m : F * m) * d;
+      w = lagIn + lagOut + C * nc * d + fl;
       break;
     }
     case K_DUAL_X:
@@ -1282,7 +1529,7 @@ int mhy_batch_create(const mhy_config* cfgs, int n, mhy_batch** out) {
   if (const char* t = std::getenv("MAHYCO_BATCH_TUNE")) b->tune = (std::string(t) != "0");
   b->ctx.resize(n, nullptr);
   for (int i = 0; i < n; ++i) {
-    int st = mhy_create(&cfgs[i], &b->ctx[i]);
+    int st = createContext(&cfgs[i], &b->ctx[i], true);
     if (st != MHY_OK)
       return batchFail(b, st, "problem " + std::to_string(i) + ": " + (b->ctx[i] ? mhy_last_error(b->ctx[i]) : ""));
     b->ctx[i]->sweepWave = true;

@@ -17,6 +17,11 @@ template <class R>
 __global__ void k_dt_hold(KPT<R> kp);
 template <class R>
 __global__ void k_lagrange_node(KPT<R> kp);
+// the same node update on the fixed Euler grid, interior and boundary nodes
+template <class R>
+__global__ void k_lagrange_node_inner(KPT<R> kp);
+template <class R>
+__global__ void k_lagrange_node_edge(KPT<R> kp);
 template <class R, int M>
 __global__ void k_lagrange_cell(KPT<R> kp);
 template <class R, bool ONE>
@@ -42,6 +47,9 @@ __global__ void k_sweep_o3_batch(BatchLaunchT<R> bl);
 dim3 sweepO3Grid(bool xd, long nx, long ny);
 template <class R, int M, bool PP, bool XD>
 __global__ void k_sweep_wave(KPT<R> kp);
+// the first sweep with the cell part of the Lagrange phase fused in
+template <class R, int M, bool PP, bool XD>
+__global__ void k_sweep_wave_lag(KPT<R> kp);
 dim3 sweepWaveGrid(bool xd, long nx, long ny, bool mixed, bool pp);
 unsigned sweepWaveBlock(bool xd, bool mixed, bool pp);
 // y sweeps with one lane per column walking down a band of rows
@@ -58,6 +66,10 @@ dim3 dualWaveGrid(bool xd, long nx, long ny, bool mixed);
 // nodal remap along y with one lane per node column walking down a band
 template <class R>
 __global__ void k_dual_walk(KPT<R> kp);
+// nodal remap along y on LDS tiles of 16 node columns x 64 rows
+template <class R>
+__global__ void k_dual_tile(KPT<R> kp);
+dim3 dualTileGrid(long nx, long ny);
 dim3 dualWalkGrid(long nx, long ny);
 unsigned dualWaveBlock(bool xd, bool mixed);
 template <class R, int M>

@@ -7,6 +7,7 @@
 // -ffp-contract=off), so FP64 results are bitwise those of the CPU path.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "lagrange_cell.hpp"
 
 namespace mhy {
 
@@ -23,8 +24,6 @@ __device__ __forceinline__ double waveMax(double v) {
   }
   return v;
 }
-
-__device__ __forceinline__ void raise(Scalars* sc, int code) { atomicCAS(&sc->err, 0, code); }
 
 // node coordinates at the step start
 template <class R>
@@ -54,7 +53,8 @@ __device__ __forceinline__ void dtReduceBody(const KPT<R>& kp, unsigned b, unsig
   const R* ux = f.ux[kp.cur];
   const R* uy = f.uy[kp.cur];
   for (long c = (long)b * blockDim.x + threadIdx.x; c < kp.ncell; c += (long)nb * blockDim.x) {
-    const long j = c / nx, i = c - j * nx;
+    // 32-bit division: cell and node counts stay below 2^31 (checked at set-up)
+    const long j = (long)((unsigned)c / (unsigned)nx), i = c - j * nx;
     const long p0 = j * (nx + 1) + i;
     const long nd[4] = {p0, p0 + 1, p0 + nx + 2, p0 + nx + 1};
     double uc = 0.0;
@@ -122,13 +122,10 @@ __global__ void k_dt_hold(KPT<R> kp) {
 // is carried over unchanged (VnrRemap.cc:152-154).  Arithmetic is FP64 in
 // both forms.
 template <class R>
-__device__ __forceinline__ void lagrangeNodeBody(const KPT<R>& kp, unsigned b) {
+__device__ __forceinline__ void lagrangeNodeAt(const KPT<R>& kp, long p, long j, long i) {
   const Opts& o = kp.o;
   const auto& f = kp.f;
-  const long p = (long)b * blockDim.x + threadIdx.x;
-  if (p >= kp.nnode) return;
   const long nx = o.nx, ny = o.ny;
-  const long j = p / (nx + 1), i = p - j * (nx + 1);
   // cells of the node in getCellsOfNode order (CartesianMesh2D.cc:97-108)
   const bool hc[4] = {j < ny && i < nx, j < ny && i > 0, j > 0 && i < nx, j > 0 && i > 0};
   const long cc[4] = {j * nx + i, j * nx + i - 1, (j - 1) * nx + i, (j - 1) * nx + i - 1};
@@ -210,195 +207,105 @@ __device__ __forceinline__ void lagrangeNodeBody(const KPT<R>& kp, unsigned b) {
 }
 
 template <class R>
+__device__ __forceinline__ void lagrangeNodeBody(const KPT<R>& kp, unsigned b) {
+  const long p = (long)b * blockDim.x + threadIdx.x;
+  if (p >= kp.nnode) return;
+  const long nx = kp.o.nx;
+  const long j = (long)((unsigned)p / (unsigned)(nx + 1)), i = p - j * (nx + 1);
+  lagrangeNodeAt(kp, p, j, i);
+}
+
+template <class R>
 __global__ __launch_bounds__(256) void k_lagrange_node(KPT<R> kp) {
   lagrangeNodeBody(kp, blockIdx.x);
 }
 
-// Cell kernel: updateCellPos, computeSubVol, updateRho (+ periodic cells),
-// computeTau, computeDivU, computeArtificialViscosity, updateEnergy,
-// computeEOS, computePressionMoyenne (PhaseLagrange.cc:214-485,
-// ConditionsLimites.cc:186-209), then the cell part of the remap
-// preparation: computeVariablesForRemap and computeCellQuantitesForRemap
-// (PrepareRemap.cc:65-176).  Arithmetic is FP64 in both forms; the mixed
-// form takes the remap masses from the FP64 material masses M_k (equal to
-// mu_k V rho in exact arithmetic), so that mass stays conserved to FP64
-// round-off whatever the FP32 density storage (periodic cells, whose
-// density is reset to its initial value, keep mu_k V rho).
-template <class R, int M>
-__device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
+// The same node update split by position, for the fixed Euler grid
+// (MODE_FULL): the interior nodes, whose four cells all exist and whose
+// corner normals are the raw half-edge sums, run in a kernel of their own
+// (blockIdx.y = node row, no index division, few registers); the boundary
+// nodes, with their conditions, run the general body above.  Each node is
+// updated by exactly one of the two, from the same operands in the same
+// order, so the results are bitwise those of k_lagrange_node.
+template <class R>
+__global__ __launch_bounds__(256) void k_lagrange_node_inner(KPT<R> kp) {
   const Opts& o = kp.o;
   const auto& f = kp.f;
+  const int nx = o.nx;
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = 1 + (int)blockIdx.y;
+  if (i >= nx) return;
+  const int p = j * (nx + 1) + i;
+  const int c0 = j * nx + i;  // cells (j, i), (j, i-1), (j-1, i), (j-1, i-1)
+  const int cc[4] = {c0, c0 - 1, c0 - nx, c0 - nx - 1};
+  double msum = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) msum = msum + f.M[cc[q]];
+  const double Mp = msum / 4.0;
+  f.Mp[p] = Mp;
+  const double uxn = (double)f.ux[kp.cur][p], uyn = (double)f.uy[kp.cur][p];
+  const Scalars* sc = f.sc;
+  const double dt1 = sc->dt[kp.nxt], dtn = sc->dt[kp.cur];
+  const double dt = 0.5 * (dt1 + dtn);
+  const D2 X0 = eulerNode(o, j, i);
+  const D2 Cn[4] = {lpcNpcAxis(X0, eulerNode(o, j, i + 1), eulerNode(o, j + 1, i)),
+                    lpcNpcAxis(X0, eulerNode(o, j + 1, i), eulerNode(o, j, i - 1)),
+                    lpcNpcAxis(X0, eulerNode(o, j - 1, i), eulerNode(o, j, i + 1)),
+                    lpcNpcAxis(X0, eulerNode(o, j, i - 1), eulerNode(o, j - 1, i))};
+  D2 r = {0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double pq = (double)f.p[kp.cur][cc[q]] + (double)f.q[kp.cur][cc[q]];
+    r = d2add(r, d2mul(Cn[q], pq));
+  }
+  const D2 u1 = d2add(D2{uxn, uyn}, d2mul(r, dt / Mp));
+  f.ux[kp.nxt][p] = u1.x;
+  f.uy[kp.nxt][p] = u1.y;
+}
+
+// the boundary nodes: bottom row, top row, then the left and right columns
+// without their corners
+template <class R>
+__global__ __launch_bounds__(256) void k_lagrange_node_edge(KPT<R> kp) {
+  const long nx = kp.o.nx, ny = kp.o.ny;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long j, i;
+  if (e < nx + 1) {
+    j = 0, i = e;
+  } else if (e < 2 * (nx + 1)) {
+    j = ny, i = e - (nx + 1);
+  } else if (e < 2 * (nx + 1) + (ny - 1)) {
+    j = 1 + (e - 2 * (nx + 1)), i = 0;
+  } else if (e < 2 * (nx + 1) + 2 * (ny - 1)) {
+    j = 1 + (e - 2 * (nx + 1) - (ny - 1)), i = nx;
+  } else {
+    return;
+  }
+  lagrangeNodeAt(kp, j * (nx + 1) + i, j, i);
+}
+
+// Cell kernel: the cell part of the Lagrange phase (lagrangeCellCompute,
+// lagrange_cell.hpp) with the remap vector, the Lagrange volume and centroid
+// (and the Lagrange density / pseudo where the options need them) stored for
+// the first remap sweep.  The wave sweeps fuse this update into their first
+// sweep instead; this kernel serves the other sweep kernels and the
+// moving-mesh mode.
+template <class R, int M>
+__device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
   const long c = (long)b * blockDim.x + threadIdx.x;
   if (c >= kp.ncell) return;
-  constexpr int m = M;  // material count (template: the loops unroll)
-  const long nx = o.nx;
-  const long j = c / nx, i = c - j * nx;
-  const int cur = kp.cur, nxt = kp.nxt;
-  const Scalars* sc = f.sc;
-  const double dt1 = sc->dt[nxt];
-  // the cell's step-start words, loaded up front: their uses below are
-  // conditional, and unconditional loads can all be in flight together
-  double fk[NMAT], Mk0[NMAT], rk0[NMAT], qk0[NMAT], ek0[NMAT], pk0[NMAT], mu0[NMAT];
-  for (int k = 0; k < m; ++k) {
-    fk[k] = f.f_k[k][c];
-    Mk0[k] = f.M_k[k][c];
-    rk0[k] = (double)f.rho_k[cur][k][c];
-    qk0[k] = (double)f.q_k[cur][k][c];
-    ek0[k] = (double)f.e_k[cur][k][c];
-    pk0[k] = (double)f.p_k[cur][k][c];
-    mu0[k] = (double)f.mu_k[k][c];
-  }
-  const double rhoS = (double)f.rho[cur][c], divuS = (double)f.divu[cur][c], cS = (double)f.c[cur][c];
-  const double vol0 = f.vol[cur][c], Mc0 = f.M[c];
-  // end-of-Lagrange node positions, node order BL, BR, TR, TL
-  D2 X[4];
-  {
-    const long jj[4] = {j, j, j + 1, j + 1}, ii[4] = {i, i + 1, i + 1, i};
-    for (int s = 0; s < 4; ++s) {
-      const long p = jj[s] * (nx + 1) + ii[s];
-      if (o.mode == MODE_LAGRANGE) {
-        X[s] = {f.X[nxt][p], f.Y[nxt][p]};
-      } else {
-        const D2 e = eulerNode(o, jj[s], ii[s]);
-        X[s] = {e.x + (double)f.ux[nxt][p] * dt1, e.y + (double)f.uy[nxt][p] * dt1};
-      }
-    }
-  }
-  D2 xc = {0.0, 0.0};
-  for (int s = 0; s < 4; ++s) xc = d2add(xc, X[s]);
-  xc = d2mul(xc, 0.25);
-  double V = 0.0;
-  for (int s = 0; s < 4; ++s) V = V + subVolume(xc, X[(s + 3) & 3], X[s], X[(s + 1) & 3]);
-
-  double rk1[NMAT];
-  double rho1 = 0.0;
-  for (int k = 0; k < m; ++k) {
-    // absent materials keep the stale buffer value in the reference; their
-    // volume fraction is exactly zero, so the value never contributes
-    rk1[k] = (fk[k] > o.threshold) ? Mk0[k] / (fk[k] * V)
-                                   : (o.mode == MODE_FULL ? 0.0 : (double)f.rho_k[nxt][k][c]);
-    rho1 += fk[k] * rk1[k];
-  }
-  const bool periodicCell = o.periodic_right && (i == 0 || i == nx - 1);
-  if (periodicCell) {
-    const long L = j * nx;  // updatePeriodicBoundaryConditions
-    rho1 = (double)f.rho0[L];
-    for (int k = 0; k < m; ++k) rk1[k] = f.rho_k0[k][L];
-  }
-
-  double q1 = 0.0, e1 = 0.0;
-  if (o.mode != MODE_ADVECTION) {
-    // mixed form: the step-start densities from the FP64 masses and volume,
-    // so that 1/rho1 - 1/rho0 (divided by dt in div u) carries no FP32
-    // rounding of the stored density
-    const double rho0 = kMixed<R> ? Mc0 / vol0 : rhoS;
-    const double tau1 = 0.5 * (1.0 / rho1 + 1.0 / rho0);
-    const double div1 = 1.0 / dt1 * (1.0 / rho1 - 1.0 / rho0) / tau1;
-    if (divuS < 0.0) {
-      const double r0 = vol0;
-      q1 = 1.0 / tau1 * (-0.5 * sqrt(r0) * cS * div1 + (o.gamma + 1) / 2.0 * r0 * div1 * div1);
-    } else {
-      q1 = 0.0;
-    }
-    const double P = sc->pmax;
-    if (q1 > P) q1 = P;
-    double qk1[NMAT], ek1[NMAT], pk1[NMAT], ck1[NMAT];
-    for (int k = 0; k < m; ++k) qk1[k] = fk[k] * q1;
-    bool bad = false;
-    for (int k = 0; k < m; ++k) {
-      ek1[k] = 0.0;
-      const double r1 = rk1[k];
-      const double r0 = (kMixed<R> && fk[k] > o.threshold) ? Mk0[k] / (fk[k] * vol0) : rk0[k];
-      if ((r1 > o.threshold) && (r0 > o.threshold)) {
-        const double qn = qk0[k];
-        double pseudo = 0.0;
-        if ((o.pseudo_centree == 1) && ((qk1[k] + qn) * (1.0 / r1 - 1.0 / r0) > 0.)) pseudo = 0.5 * (qk1[k] + qn);
-        if (o.pseudo_centree == 0) pseudo = qk1[k];
-        const double den = 1 + 0.5 * (o.gammap[k] - 1.0) * r1 * (1.0 / r1 - 1.0 / r0);
-        const double num = ek0[k] - (0.5 * pk0[k] + pseudo) * (1.0 / r1 - 1.0 / r0);
-        ek1[k] = num / den;
-        e1 += mu0[k] * ek1[k];
-        if (fabs(ek1[k]) > 5. && o.abort_energy) bad = true;
-      }
-    }
-    if (bad) raise(f.sc, MHY_ERR_ENERGY);
-    // computeEOS: perfect gas and void (PhaseLagrange.cc:392-430)
-    for (int k = 0; k < m; ++k) {
-      const double g = o.gammap[k];
-      if (o.eos[k] == MHY_EOS_PERFECT_GAS) {
-        pk1[k] = (g - 1.0) * rk1[k] * ek1[k];
-        ck1[k] = sqrt(g * (g - 1.0) * ek1[k]);
-      } else if (o.eos[k] == MHY_EOS_VOID) {
-        pk1[k] = pk0[k];
-        ck1[k] = f.c_k[cur][k][c];
-        ek1[k] = ek0[k];
-        e1 = 0.0;
-      } else {
-        pk1[k] = f.p_k[nxt][k][c];
-        ck1[k] = f.c_k[nxt][k][c];
-      }
-    }
-    // computePressionMoyenne (PhaseLagrange.cc:464-485)
-    double p1 = 0.0, c1 = 0.0;
-    const bool needStale = !(rho1 > 0.);
-    if (needStale) c1 = (double)f.c[nxt][c];
-    for (int k = 0; k < m; ++k) {
-      p1 += fk[k] * pk1[k];
-      if (needStale) c1 = smax(c1, ck1[k]);
-    }
-    if (rho1 > 0.) c1 = sqrt(o.gammap[0] * (o.gammap[0] - 1.0) * e1);
-    for (int k = 0; k < m; ++k)
-      if (o.eos[k] == MHY_EOS_VOID) e1 += mu0[k] * ek1[k];
-    f.divu[nxt][c] = div1;
-    for (int k = 0; k < m; ++k) f.q_k[nxt][k][c] = qk1[k];
-    if (o.mode == MODE_LAGRANGE) {
-      f.rho[nxt][c] = rho1;
-      f.q[nxt][c] = q1;
-      f.e[nxt][c] = e1;
-      f.p[nxt][c] = p1;
-      f.c[nxt][c] = c1;
-      for (int k = 0; k < m; ++k) {
-        f.rho_k[nxt][k][c] = rk1[k];
-        f.e_k[nxt][k][c] = ek1[k];
-        f.p_k[nxt][k][c] = pk1[k];
-        f.c_k[nxt][k][c] = ck1[k];
-      }
-      f.vol[nxt][c] = V;  // node_cellvolume_n swap (VnrRemap.cc:217-221)
-      return;
-    }
-  } else {
-    // advection: the energy swap of VnrRemap.cc:170 makes the remap read the
-    // current energy; the pseudo buffer of the n+1 slot is read as left by
-    // the previous remap of that slot
-    e1 = (double)f.e[cur][c];
-    q1 = (double)f.q[nxt][c];
-  }
-  // computeVariablesForRemap (PrepareRemap.cc:96-176), live components only
-  for (int k = 0; k < m; ++k) {
-    f.UL[k][c] = fk[k] * V;
-    if (kMixed<R> && !periodicCell) {
-      const double Mk = Mk0[k];
-      f.UL[m + k][c] = Mk;
-      f.UL[2 * m + k][c] = Mk * e1;
-    } else {
-      const double mu = mu0[k];
-      f.UL[m + k][c] = mu * V * rho1;
-      f.UL[2 * m + k][c] = mu * V * rho1 * e1;
-    }
-  }
-  f.UL[3 * m][c] = q1 * V;
-  f.VL[c] = V;
-  if (o.methode == 1) f.rL[c] = rho1;
-  if (o.pp == 1) f.qL[c] = q1;
-  // computeCellQuantitesForRemap (PrepareRemap.cc:65-90): polygon centroid
-  D2 r = {0.0, 0.0};
-  for (int s = 0; s < 4; ++s) {
-    const D2 a = X[s], b = X[(s + 1) & 3];
-    r = d2add(r, d2mul(d2add(a, b), d2cross(a, b)));
-  }
-  r = d2mul(r, 1.0 / (6.0 * V));
-  f.XcL[c] = r.x;
-  f.YcL[c] = r.y;
+  const long nx = kp.o.nx;
+  const long j = (long)((unsigned)c / (unsigned)nx), i = c - j * nx;
+  LagCell<M> lc;
+  if (!lagrangeCellCompute<R, M>(kp, c, j, i, true, lc)) return;
+  const auto& f = kp.f;
+#pragma unroll
+  for (int q = 0; q < 3 * M + 1; ++q) f.UL[q][c] = lc.U[q];
+  f.VL[c] = lc.V;
+  if (kp.o.methode == 1) f.rL[c] = lc.rho;
+  if (kp.o.pp == 1) f.qL[c] = lc.q;
+  f.XcL[c] = lc.xc.x;
+  f.YcL[c] = lc.xc.y;
 }
 
 template <class R, int M>
@@ -447,6 +354,8 @@ __global__ __launch_bounds__(256) void k_lagrange_cell_batch(BatchLaunchT<R> bl)
   template __global__ void k_dt_finalize_batch<R>(BatchLaunchT<R>);          \
   template __global__ void k_dt_hold<R>(KPT<R>);                             \
   template __global__ void k_lagrange_node<R>(KPT<R>);                       \
+  template __global__ void k_lagrange_node_inner<R>(KPT<R>);                 \
+  template __global__ void k_lagrange_node_edge<R>(KPT<R>);                  \
   template __global__ void k_lagrange_cell<R, 1>(KPT<R>);                    \
   template __global__ void k_lagrange_cell<R, 2>(KPT<R>);                    \
   template __global__ void k_lagrange_cell<R, 3>(KPT<R>);                    \

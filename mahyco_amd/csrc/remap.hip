@@ -525,9 +525,130 @@ dim3 dualWalkGrid(long nx, long ny) {
   return dim3((unsigned)((gx * bands + 3) / 4));
 }
 
+// Nodal remap along y on an LDS tile: a 256-thread block covers 16 node
+// columns and 64 node rows (3 halo rows above and below, 58 rows updated),
+// each thread 4 rows of one column, so that every load and store instruction
+// of a wave touches 4 whole row segments of 16 nodes (128 bytes in FP64).
+// The block loads DualPhi and the Lagrange coordinate of the tile into LDS,
+// forms each row's limited gradient from its neighbours there, and updates
+// the inner rows; every value comes from the same operands as in the
+// wave-strip kernel, so results are bitwise the same.
+namespace {
+constexpr int kDTCols = 16;                     // node columns per tile
+constexpr int kDTRowsPer = 4;                   // rows per thread
+constexpr int kDTRows = 256 / kDTCols * kDTRowsPer;  // rows per tile (64)
+constexpr int kDTOut = kDTRows - 6;             // rows updated (58)
+}  // namespace
+
+template <class R>
+__device__ __forceinline__ void dualTileBody(const KPT<R>& kp, unsigned b) {
+  using T = typename std::conditional<kMixed<R>, float, double>::type;
+  const Opts& o = kp.o;
+  const auto& f = kp.f;
+  const long nx = o.nx, na = o.ny, nt = o.nx;
+  const long gx = (nx + 1 + kDTCols - 1) / kDTCols;
+  const long bx = b % gx, by = b / gx;
+  const int col = threadIdx.x % kDTCols, rg = threadIdx.x / kDTCols;
+  const long t = bx * kDTCols + col;
+  const long r0 = by * kDTOut - 3;  // node row of tile row 0
+  const bool inT = (t <= nt);
+  const double dt1 = f.sc->dt[kp.nxt];
+  __shared__ T sdx[kDTRows][kDTCols], sdy[kDTRows][kDTCols], sgx[kDTRows][kDTCols], sgy[kDTRows][kDTCols];
+  __shared__ double sxl[kDTRows][kDTCols];
+  // tile row of the k-th row of this thread: 16 consecutive rows per pass
+  auto tileRow = [&](int k) { return rg + k * (256 / kDTCols); };
+  auto inRow = [&](int L) { return (r0 + L >= 0) && (r0 + L <= na); };
+  // own DualPhi and Lagrange coordinate of the tile's nodes
+#pragma unroll
+  for (int k = 0; k < kDTRowsPer; ++k) {
+    const int L = tileRow(k);
+    const long a = r0 + L;
+    T dx = T(0.0), dy = T(0.0);
+    double xl = 0.0;
+    if (inRow(L) && inT) {
+      const long p = a * (nx + 1) + t;
+      if (kp.sweep == 1) {
+        const double ux1 = (double)f.ux[kp.nxt][p], uy1 = (double)f.uy[kp.nxt][p];
+        dx = T(ux1), dy = T(uy1);
+        const D2 e = eulerNode(o, a, t);
+        xl = e.y + uy1 * dt1;
+        // Lagrange coordinate along the second sweep's axis (the second sweep
+        // overwrites the velocity in place)
+        if (L >= 3 && L < 3 + kDTOut) f.XL2[p] = e.x + ux1 * dt1;
+      } else {
+        const double M1 = f.UD1[2][p];
+        dx = T(f.UD1[0][p] / M1), dy = T(f.UD1[1][p] / M1);
+        xl = f.XL2[p];
+      }
+    }
+    sdx[L][col] = dx, sdy[L][col] = dy, sxl[L][col] = xl;
+  }
+  __syncthreads();
+  // limited gradients of the rows the updates need (tile rows 2 .. 61)
+#pragma unroll
+  for (int k = 0; k < kDTRowsPer; ++k) {
+    const int L = tileRow(k);
+    if (L < 2 || L >= kDTRows - 2) continue;
+    T g0x = T(0.0), g0y = T(0.0);
+    if (inT && o.order > 1 && inRow(L) && inRow(L - 1) && inRow(L + 1)) {
+      // divided differences towards the row above, at rows L and L+1
+      const T glx = (sdx[L][col] - sdx[L - 1][col]) / T(sxl[L][col] - sxl[L - 1][col]);
+      const T gly = (sdy[L][col] - sdy[L - 1][col]) / T(sxl[L][col] - sxl[L - 1][col]);
+      const T grx = (sdx[L + 1][col] - sdx[L][col]) / T(sxl[L + 1][col] - sxl[L][col]);
+      const T gry = (sdy[L + 1][col] - sdy[L][col]) / T(sxl[L + 1][col] - sxl[L][col]);
+      const T h0 = T(0.5 * (sxl[L + 1][col] - sxl[L - 1][col]));
+      T hm, hp;
+      if (!inRow(L - 2)) {
+        hm = T(0.), hp = T(0.5 * (sxl[L + 2][col] - sxl[L][col]));
+      } else if (!inRow(L + 2)) {
+        hp = T(0.), hm = T(0.5 * (sxl[L][col] - sxl[L - 2][col]));
+      } else {
+        hm = T(0.5 * (sxl[L][col] - sxl[L - 2][col]));
+        hp = T(0.5 * (sxl[L + 2][col] - sxl[L][col]));
+      }
+      g0x = sw::limitedGradient<T>(o.limiter, grx, glx, sdx[L + 1][col], sdx[L - 1][col], h0, hp, hm);
+      g0y = sw::limitedGradient<T>(o.limiter, gry, gly, sdy[L + 1][col], sdy[L - 1][col], h0, hp, hm);
+    }
+    sgx[L][col] = g0x, sgy[L][col] = g0y;
+  }
+  __syncthreads();
+  // updates of the inner rows
+#pragma unroll
+  for (int k = 0; k < kDTRowsPer; ++k) {
+    const int L = tileRow(k);
+    if (L < 3 || L >= 3 + kDTOut) continue;
+    const long a = r0 + L;
+    if (!inT || !inRow(L)) continue;
+    const T dpx[3] = {sdx[L - 1][col], sdx[L][col], sdx[L + 1][col]};
+    const T dpy[3] = {sdy[L - 1][col], sdy[L][col], sdy[L + 1][col]};
+    const double xls[3] = {sxl[L - 1][col], sxl[L][col], sxl[L + 1][col]};
+    const T g0[2] = {sgx[L][col], sgy[L][col]};
+    T gM[2] = {sgx[L - 1][col], sgy[L - 1][col]}, gP[2] = {sgx[L + 1][col], sgy[L + 1][col]};
+    // a neighbour outside the domain has a zero gradient
+    if (!inRow(L - 1)) gM[0] = gM[1] = T(0.0);
+    if (!inRow(L + 1)) gP[0] = gP[1] = T(0.0);
+    dualNodeFinish<R, false, T>(kp, a, t, na, nt, a * (nx + 1) + t, dpx, dpy, xls, g0, gM, gP);
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_dual_tile(KPT<R> kp) {
+  // neighbouring column groups share the cache lines of the face fluxes
+  // left of their first column: one XCD (one L2) takes a contiguous range
+  dualTileBody<R>(kp, xcdBlock());
+}
+
+dim3 dualTileGrid(long nx, long ny) {
+  const long gx = (nx + 1 + kDTCols - 1) / kDTCols, gy = (ny + 1 + kDTOut - 1) / kDTOut;
+  return dim3((unsigned)(gx * gy));
+}
+
 template <class R, bool XD>
 __global__ __launch_bounds__(XD ? 256 : 64 * kDYWaves<R>) void k_dual_wave(KPT<R> kp) {
-  dualWaveBody<R, XD>(kp, blockIdx.x);
+  // along y, neighbouring column groups share the cache lines of the face
+  // fluxes left of their first column: one XCD (one L2) takes a contiguous
+  // range of blocks
+  dualWaveBody<R, XD>(kp, XD ? blockIdx.x : xcdBlock());
 }
 template <class R, bool XD, bool ONE>
 __global__ __launch_bounds__(XD ? 256 : 64 * kDYWaves<R>) void k_dual_wave_batch(BatchLaunchT<R> bl) {
@@ -642,7 +763,8 @@ __global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
   template __global__ void k_remap_final<R, 2>(KPT<R>);                \
   template __global__ void k_remap_final<R, 3>(KPT<R>);                \
   template __global__ void k_node_mass<R>(KPT<R>);                     \
-  template __global__ void k_conservation<R>(KPT<R>, double*);
+  template __global__ void k_conservation<R>(KPT<R>, double*);         \
+  template __global__ void k_dual_tile<R>(KPT<R>);
 MHY_REMAP_INST(double)
 MHY_REMAP_INST(float)
 #undef MHY_REMAP_INST

@@ -25,16 +25,20 @@ __device__ __forceinline__ void twoSumAcc(double& s, double& comp, double v) {
 // its minus / plus faces (valid when hasM / hasP), face normal velocities
 // vm / vp and Euler face lengths Lm / Lp: its new remap vector Unew and the
 // flux of its minus face for the dual remap (A1: mass, A2: volume; zero for
-// PB).  The flux of the last face of a line is stored here.
+// PB).  The flux of the last face of a line is stored here.  Uown: the
+// cell's remap vector at the start of the sweep when the caller holds it in
+// registers (the first sweep with the Lagrange update fused), else null and
+// it is read from U_L / U_1.
 template <class R, int M, bool PP, bool XD, class T>
 __device__ __forceinline__ void sweepUnew(const KPT<R>& kp, long c, long i, long j, long a, bool hasM, bool hasP,
                                           double vm, double vp, double Lm, double Lp, const T* phiM, const T* phiP,
-                                          double* Unew, double* Gdual) {
+                                          double* Unew, double* Gdual, const double* Uown = nullptr) {
   constexpr int NC = 3 * M + 1;
   const Opts& o = kp.o;
   const auto& f = kp.f;
   const long nx = o.nx, ny = o.ny;
   double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
+  auto uOld = [&](int q) { return Uown ? Uown[q] : Uin[q][c]; };
   const double dtn = f.sc->dt[kp.cur];
   // computeUremap (PhaseRemap1.cc:468-503): a cell receives -G through its
   // minus face and +G through its plus face
@@ -80,7 +84,7 @@ __device__ __forceinline__ void sweepUnew(const KPT<R>& kp, long c, long i, long
     // rounded sum up to a final rounding, whatever the flux magnitudes
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
-      double sum = Uin[q][c], comp = 0.0;
+      double sum = uOld(q), comp = 0.0;
       if (hasM) twoSumAcc(sum, comp, Gm[q]);
       if (hasP) twoSumAcc(sum, comp, -Gp[q]);
       twoSumAcc(sum, comp, -Fb[q]);
@@ -88,7 +92,7 @@ __device__ __forceinline__ void sweepUnew(const KPT<R>& kp, long c, long i, long
     }
   } else {
 #pragma unroll
-    for (int q = 0; q < NC; ++q) Unew[q] = Uin[q][c] - red[q];
+    for (int q = 0; q < NC; ++q) Unew[q] = uOld(q) - red[q];
   }
   const bool withG = (o.methode == 0 || o.methode == 1);
   const int q0 = (o.methode == 0) ? M : 0;
@@ -107,11 +111,11 @@ __device__ __forceinline__ void sweepUnew(const KPT<R>& kp, long c, long i, long
 template <class R, int M, bool PP, bool XD, class T>
 __device__ __forceinline__ void sweepUpdateCell(const KPT<R>& kp, long c, long i, long j, long a, bool hasM,
                                                 bool hasP, double vm, double vp, double Lm, double Lp,
-                                                const T* phiM, const T* phiP) {
+                                                const T* phiM, const T* phiP, const double* Uown = nullptr) {
   constexpr int NC = 3 * M + 1;
   const auto& f = kp.f;
   double Unew[NC], Gd[M];
-  sweepUnew<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, Gd);
+  sweepUnew<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, Gd, Uown);
   if (kp.fuseFinal) {
     finalCell<R, M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
   } else {

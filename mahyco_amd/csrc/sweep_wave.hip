@@ -21,10 +21,18 @@
 // arithmetic, because their swept integrals of nearly emptied materials
 // cancel catastrophically in FP32.  Each face flux is formed in FP64 and
 // accumulated into the FP64 remap vector with a compensated (TwoSum) update.
+//
+// LAG: the first sweep of a step also runs the cell part of the Lagrange
+// phase (lagrangeCellCompute): each lane forms its cell's Lagrange update and
+// remap vector in registers, the halo lanes redundantly, and only the
+// updated cells store the words that outlive the step.  The remap vector,
+// the Lagrange volume and the centroid never go through memory; the second
+// sweep forms the volume and centroid again from the node velocities.
 #include <type_traits>
 
 #include "final_math.hpp"
 #include "kernels.hpp"
+#include "lagrange_cell.hpp"
 #include "sweep_math.hpp"
 #include "sweep_update.hpp"
 
@@ -83,7 +91,8 @@ template <class R, int M, bool PP, class T, bool FUSE>
 __device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, long j, long a, long bx, long by,
                                              int slot, int wave, int lane, bool output, bool hasM, bool hasP,
                                              double vm, double vp, double Lm, double Lp, const T* phiM,
-                                             const T* phiP, const double* dfv, bool pb, double* stage) {
+                                             const T* phiP, const double* dfv, bool pb, double* stage,
+                                             const double* Uown) {
   constexpr int NC = 3 * M + 1;
   const Opts& o = kp.o;
   const auto& f = kp.f;
@@ -96,7 +105,7 @@ __device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, l
   if (output) {
     CellOut<R, M> out;
     double Unew[NC];
-    sweepUnew<R, M, PP, false, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, out.G);
+    sweepUnew<R, M, PP, false, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, out.G, Uown);
     if constexpr (FUSE) {
       finalCompute<R, M>(kp, c, Unew, out.fin);
     } else {
@@ -128,7 +137,7 @@ __device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, l
 // b: linear block index.  Along x the strips of all rows are numbered
 // row-major and each wave of the block takes the next one; along y a block
 // covers 16 columns x 12 rows, blocks ordered column-group-major per row band.
-template <class R, int M, bool PP, bool XD>
+template <class R, int M, bool PP, bool XD, bool LAG>
 __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   // type of the slope / limiter / PenteBorne arithmetic
   using T = typename std::conditional<kMixed<R> && !PP, float, double>::type;
@@ -173,16 +182,45 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   const bool mixteOpt = PP && o.pp_mixte == 1;
   const bool pb = PP && o.methode == 2;
 
+  // the cell's remap vector at the sweep start, kept when the Lagrange
+  // update forms it in registers (no copy in memory)
+  double uOwn[NC];
   // one cell of this lane's line at along index b: Phi, centroid, width, flags
   auto loadCell = [&](long b, T* ph, double& x, T& h, int& flg) {
     const long jb = XD ? t : b, ib = XD ? b : t;
     const long cb = jb * nx + ib;
     double u[NC];
-    const int nload = PP ? 3 * M : NC;
+    double V, qv = 0.0;
+    if constexpr (LAG) {
+      // the Lagrange update of the cell, its remap vector in registers; only
+      // an updated cell stores what outlives the step
+      LagCell<M> lc;
+      lagrangeCellCompute<R, M>(kp, cb, jb, ib, output, lc);
 #pragma unroll
-    for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cb] : 0.0;
-    cellPhi<M, PP, T>(u, PP ? 1.0 : f.VL[cb], PP ? f.qL[cb] : 0.0, ph);
-    x = XD ? f.XcL[cb] : f.YcL[cb];
+      for (int q = 0; q < NC; ++q) u[q] = uOwn[q] = lc.U[q];
+      V = lc.V, qv = lc.q;
+      x = XD ? lc.xc.x : lc.xc.y;
+      if (output && o.methode == 1) f.rL[cb] = lc.rho;  // A2 dual fluxes
+      if (output && PP) f.qL[cb] = lc.q;                // PenteBorne Phi of the second sweep
+      if (output && kp.geomLoad) f.VL[cb] = lc.V, f.XcL[cb] = lc.xc.x, f.YcL[cb] = lc.xc.y;
+    } else {
+      const int nload = PP ? 3 * M : NC;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cb] : 0.0;
+      // the Lagrange volume and centroid: stored by the cell update, or
+      // formed again from the end-of-Lagrange node positions (bitwise the
+      // values the cell update formed)
+      if (kp.geomLoad) {
+        V = f.VL[cb];
+        x = XD ? f.XcL[cb] : f.YcL[cb];
+      } else {
+        D2 xcv;
+        lagGeometry(kp, dt1, jb, ib, V, xcv);
+        x = XD ? xcv.x : xcv.y;
+      }
+      if (PP) qv = f.qL[cb];
+    }
+    cellPhi<M, PP, T>(u, PP ? 1.0 : V, qv, ph);
     h = needH ? T(widthW<XD>(kp, dt1, jb, ib)) : T(0.0);
     flg = 0;
     if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<M, T>(ph);
@@ -313,16 +351,17 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 #pragma unroll
       for (int k = 0; k < M; ++k) f.DF[k][c] = dfv[k];
     }
-    sweepUpdateCell<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP);
+    sweepUpdateCell<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, LAG ? uOwn : nullptr);
   } else {
     // one buffer for both sweep orders: kStageWords doubles per tile cell
     __shared__ double stage[kStageWords<R, M, PP> * kYOut * kYWaves<R, PP> * kYCols];
+    const double* Uown = LAG ? uOwn : nullptr;
     if (kp.fuseFinal)
       sweepYStaged<R, M, PP, T, true>(kp, c, i, j, a, bx, by, slot, wave, lane, output, hasM, hasP, vm, vp, Lm, Lp,
-                                      phiM, phiP, dfv, pb, stage);
+                                      phiM, phiP, dfv, pb, stage, Uown);
     else
       sweepYStaged<R, M, PP, T, false>(kp, c, i, j, a, bx, by, slot, wave, lane, output, hasM, hasP, vm, vp, Lm, Lp,
-                                       phiM, phiP, dfv, pb, stage);
+                                       phiM, phiP, dfv, pb, stage, Uown);
   }
 }
 
@@ -339,7 +378,14 @@ constexpr int kMinWavesBatch = (M < 3) ? kMinWaves<R, PP, XD> : 1;
 template <class R, int M, bool PP, bool XD>
 __global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>))
 __attribute__((amdgpu_waves_per_eu(kMinWaves<R, PP, XD>))) void k_sweep_wave(KPT<R> kp) {
-  sweepWaveBody<R, M, PP, XD>(kp, XD ? blockIdx.x : xcdBlock());
+  sweepWaveBody<R, M, PP, XD, false>(kp, XD ? blockIdx.x : xcdBlock());
+}
+
+// the first sweep of a step with the cell part of the Lagrange phase
+template <class R, int M, bool PP, bool XD>
+__global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>))
+__attribute__((amdgpu_waves_per_eu(kMinWaves<R, PP, XD>))) void k_sweep_wave_lag(KPT<R> kp) {
+  sweepWaveBody<R, M, PP, XD, true>(kp, XD ? blockIdx.x : xcdBlock());
 }
 
 // all problems of an ensemble in one launch (BatchLaunchT, common.hpp)
@@ -347,13 +393,15 @@ template <class R, int M, bool PP, bool XD, bool ONE>
 __global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>))
 __attribute__((amdgpu_waves_per_eu(kMinWavesBatch<R, M, PP, XD>))) void k_sweep_wave_batch(BatchLaunchT<R> bl) {
   batchRun<ONE>(bl, [&](const KPT<R>& kp, unsigned local, unsigned, unsigned) {
-    sweepWaveBody<R, M, PP, XD>(kp, local);
+    sweepWaveBody<R, M, PP, XD, false>(kp, local);
   });
 }
 
 #define MHY_SWEEP_WAVE_INST(R, M, PP)                                          \
   template __global__ void k_sweep_wave<R, M, PP, true>(KPT<R>);               \
   template __global__ void k_sweep_wave<R, M, PP, false>(KPT<R>);              \
+  template __global__ void k_sweep_wave_lag<R, M, PP, true>(KPT<R>);           \
+  template __global__ void k_sweep_wave_lag<R, M, PP, false>(KPT<R>);          \
   template __global__ void k_sweep_wave_batch<R, M, PP, true, false>(BatchLaunchT<R>);  \
   template __global__ void k_sweep_wave_batch<R, M, PP, false, false>(BatchLaunchT<R>); \
   template __global__ void k_sweep_wave_batch<R, M, PP, true, true>(BatchLaunchT<R>);   \

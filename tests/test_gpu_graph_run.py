@@ -58,7 +58,11 @@ def test_run_with_kernel_timing_is_eager_and_equal():
         g.enable_timing(False)
         g.run(3)  # graphs again
         e.step(11)
-        assert all(v["launches"] == 4 for k, v in stats.items() if k.startswith("sweep")), stats
+        # each direction is swept twice per step: the first sweep (with the
+        # Lagrange cell update fused in, its own kernel id) and the second
+        for d in ("x", "y"):
+            n = sum(v["launches"] for k, v in stats.items() if k.endswith(f"sweep_{d}"))
+            assert n == 4, stats
         diff = _differing(g, e, cfg)
         assert not diff, diff
     finally:

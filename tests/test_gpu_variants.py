@@ -1,10 +1,15 @@
 """The kernel variants that the environment selects must give bitwise the
 state of the default kernels: the per-cell stencil sweeps
 (MAHYCO_SWEEP_KERNEL=cell, FP64 form only), the separate final-remap kernel
-(MAHYCO_FUSE_FINAL=0), the per-node dual remap along y (MAHYCO_DUAL_Y=node,
+(MAHYCO_FUSE_FINAL=0), the separate Lagrange cell kernel instead of its
+fusion into the first sweep (MAHYCO_FUSE_LAGRANGE=0), the per-node dual
+remap along y (MAHYCO_DUAL_Y=node,
 FP64 form: in the mixed form it keeps FP64 velocity arithmetic) and the
-column-walking dual remap along y (MAHYCO_DUAL_Y=walk), in both storage
-forms.  The choice is read when a context is created."""
+column-walking and LDS-tile dual remaps along y (MAHYCO_DUAL_Y=walk,
+tile), the Lagrange volume and centroid formed again in the second sweep or
+stored by the first (MAHYCO_SWEEP_GEOM=recompute, load), and one node
+kernel for interior and boundary nodes (MAHYCO_SPLIT_NODE=0), in both
+storage forms.  The choice is read when a context is created."""
 import os
 from contextlib import contextmanager
 
@@ -19,8 +24,11 @@ from test_gpu_parity import extension_cases
 pytestmark = pytest.mark.gpu
 STEPS = 12
 VARIANTS = {"cell_sweeps": ("MAHYCO_SWEEP_KERNEL", "cell"), "separate_final": ("MAHYCO_FUSE_FINAL", "0"),
+            "separate_lagrange": ("MAHYCO_FUSE_LAGRANGE", "0"),
             "node_dual_y": ("MAHYCO_DUAL_Y", "node"), "walk_dual_y": ("MAHYCO_DUAL_Y", "walk"),
-            "wave_dual_y": ("MAHYCO_DUAL_Y", "wave")}
+            "wave_dual_y": ("MAHYCO_DUAL_Y", "wave"), "tile_dual_y": ("MAHYCO_DUAL_Y", "tile"),
+            "recomputed_geometry": ("MAHYCO_SWEEP_GEOM", "recompute"),
+            "stored_geometry": ("MAHYCO_SWEEP_GEOM", "load"), "one_node_kernel": ("MAHYCO_SPLIT_NODE", "0")}
 NAMES = ["C2_pp_superbee", "C2_pp_PB", "C2_pp_mixte_arithmeticG", "C3_point_explosion", "C4_triple_layout",
          "riemann_order1", "riemann_superBeeG_main", "unit_test_boundary_flux", "riemann_order3"]
 
