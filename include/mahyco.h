@@ -43,6 +43,8 @@ extern "C" {
 #define MHY_ERR_NAN -5         /* NaN after remap (PhaseRemapFinal.cc:174-196) */
 #define MHY_ERR_UNKNOWN_CASE -6 /* Init.cc:368-371                           */
 #define MHY_ERR_FINISHED -7    /* time loop already ended                   */
+#define MHY_ERR_DT_MIN -8      /* Eucclhyd time step below deltat_min
+                                  (eucclhyd_remap/EucclhydRemap.cc:170-176) */
 
 /* option values (includes/Limiteurs.h, Eos.h, ConditionsLimites.h,
  * SchemaLagrange.h, CasTest.h) */
@@ -154,6 +156,10 @@ typedef struct mhy_config {
 #define MHY_CELL_MASS 6         /* m_cell_mass              */
 #define MHY_CELL_MIXTE 7        /* varlp->mixte (as double) */
 #define MHY_CELL_PURE 8         /* varlp->pure  (as double) */
+#define MHY_CELL_VELOCITY_X 9   /* Eucclhyd m_cell_velocity_n[0] */
+#define MHY_CELL_VELOCITY_Y 10  /* Eucclhyd m_cell_velocity_n[1] */
+#define MHY_CELL_OUT_VX 11      /* Eucclhyd m_x_cell_velocity (output) */
+#define MHY_CELL_OUT_VY 12      /* Eucclhyd m_y_cell_velocity (output) */
 /* per-material cell fields: add MHY_MAT_STRIDE*k to the id */
 #define MHY_MAT_FRACVOL 16      /* m_fracvol_env            */
 #define MHY_MAT_MASS_FRACTION 17 /* m_mass_fraction_env     */

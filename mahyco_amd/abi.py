@@ -16,11 +16,12 @@ MHY_ERR_ENERGY = -4
 MHY_ERR_NAN = -5
 MHY_ERR_UNKNOWN_CASE = -6
 MHY_ERR_FINISHED = -7
+MHY_ERR_DT_MIN = -8
 
 LIM = {"minmod": 300, "superBee": 301, "vanLeer": 302, "minmodG": 1300, "superBeeG": 1301,
        "vanLeerG": 1302, "arithmeticG": 1303, "ultrabeeG": 1304}
 EOS_VOID, EOS_PERFECT_GAS = 100, 101
-SCHEMA_VNR = 2001
+SCHEMA_EUCCLHYD, SCHEMA_VNR = 2000, 2001
 
 CASE = {"UnitTestCase": 0, "SedovTestCase": 1, "TriplePoint": 2, "SodCaseX": 4, "SodCaseY": 5,
         "NohTestCase": 6, "AdvectionX": 7, "AdvectionY": 8, "AdvectionVitX": 9, "AdvectionVitY": 10,
@@ -40,7 +41,9 @@ NODE_COORD_X, NODE_COORD_Y, NODE_OUT_VX, NODE_OUT_VY = 67, 68, 69, 70
 
 CELL_FIELDS = {"density": CELL_DENSITY, "pressure": CELL_PRESSURE, "energy": CELL_ENERGY,
                "pseudo": CELL_PSEUDO, "sound_speed": CELL_SOUND_SPEED, "divu": CELL_DIVU,
-               "cell_mass": CELL_MASS, "mixte": CELL_MIXTE, "pure": CELL_PURE}
+               "cell_mass": CELL_MASS, "mixte": CELL_MIXTE, "pure": CELL_PURE,
+               # Eucclhyd only
+               "cell_velocity_x": 9, "cell_velocity_y": 10, "cell_out_vx": 11, "cell_out_vy": 12}
 MAT_FIELDS = {"fracvol": MAT_FRACVOL, "mass_fraction": MAT_MASS_FRACTION,
               "density_env": MAT_DENSITY, "energy_env": MAT_ENERGY, "pressure_env": MAT_PRESSURE,
               "pseudo_env": MAT_PSEUDO, "mass_env": MAT_MASS, "sound_speed_env": MAT_SOUND_SPEED}

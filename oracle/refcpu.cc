@@ -1,19 +1,22 @@
 /*
  * ORACLE -- test infrastructure, not product code.
  *
- * CPU restatement of MaHyCo's VNR Lagrange + directional remap time loop:
- * the same arithmetic, in the same operation order, on the same n/n+1
- * buffer swaps as the reference, but with closed-form structured indexing
- * instead of the reference's O(N) mesh queries.  Each function cites the
- * reference file:line it restates.  Built with g++ -ffp-contract=off so no
+ * CPU restatement of MaHyCo's VNR Lagrange + directional remap time loop,
+ * and of the Eucclhyd Lagrange scheme that calls the same cell sweeps: the
+ * same arithmetic, in the same operation order, on the same n/n+1 buffer
+ * swaps as the reference, but with closed-form structured indexing instead
+ * of the reference's O(N) mesh queries.  Each function cites the reference
+ * file:line it restates.  Built with g++ -ffp-contract=off so no
  * multiply-add is fused (the reference is compiled for x86-64 without FMA).
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load this library, as the checker; the product never calls it.
  *
- * Pinned by: the 37 VNR golden VTP fixtures of the reference (tests/<case>/
- * reference/ VTP files, compared as text), plus invariants (mass conservation of
- * the sweeps, uniform-state preservation, X/Y mirror symmetry) in tests/.
+ * Pinned by: the 63 golden VTP fixtures of the reference (37 VNR and 26
+ * Eucclhyd tests/<case>/reference/ VTP files, compared as text), plus
+ * closed forms and invariants (limiters, swept integrals, PenteBorne-mixte
+ * profiles, mass conservation of the sweeps, uniform-state preservation,
+ * X/Y mirror symmetry) in tests/.
  */
 #include "refcpu.h"
 
@@ -307,7 +310,64 @@ struct ref_ctx {
   int remapVariables();
   int iterate(bool dump, bool* continueLoop);
   void dumpVariables();
+  void writeVtp(const std::string& module, long N, const V2* nodes, long C,
+                const std::map<std::string, const double*>& cellVars,
+                const std::map<std::string, const double*>& nodeVars, std::map<double, std::string>& byTimes);
   void swapAll();
+
+  // Eucclhyd fields (eucclhyd_remap/Eucclhyd.h:86-160), allocated for that
+  // scheme only.  The reference keeps the corner quantities per node and
+  // cell of the node; here they are kept per cell corner, slot s in the
+  // quad node order BL, BR, TR, TL (one-to-one, same values).
+  typedef std::array<double, 4> M22;  // row-major 2x2
+  std::vector<std::array<V2, 4>> e_lpc, e_nplus, e_nminus;
+  std::vector<V4> e_lplus, e_lminus;
+  std::vector<double> e_perimeter, e_cell_deltat, x_cell_velocity, y_cell_velocity;
+  std::vector<V2> cell_velocity_n, cell_velocity_nplus1, cell_velocity_n0;
+  std::vector<M22> e_velgrad;
+  std::vector<std::array<V2, NMAT>> e_pgrad_env;
+  std::vector<V4> e_p_extrap;
+  std::vector<std::array<VM, 4>> e_pe_extrap;
+  std::vector<std::array<V2, 4>> e_v_extrap, e_F_n, e_F_nplus1;
+  std::vector<std::array<std::array<V2, NMAT>, 4>> e_Fenv_n, e_Fenv_nplus1;
+  std::vector<std::array<M22, 4>> e_M;
+  std::vector<std::array<std::array<M22, NMAT>, 4>> e_Menv;
+  std::vector<M22> e_nodeM;
+  std::vector<V2> e_nodeG, XfLagrange;
+  std::vector<VM> lagrange_volume;
+  std::map<double, std::string> particleFileNameByTimes;
+
+  bool euc() const { return o.schema == MHY_SCHEMA_EUCCLHYD; }
+  void eucAllocate();
+  int eucPrologue();
+  void eucInitBoundaryConditions();
+  void eucCornerNormals();
+  void eucInitMeshGeometryForCells();
+  int eucInitState();
+  void eucSetUpTimeLoopN();
+  void eucComputeEOS();
+  void eucPressionMoyenne();
+  void eucGradients();
+  void eucMass();
+  void eucDissipationMatrix();
+  void eucCellDeltat();
+  void eucExtrapolate();
+  void eucNodeG();
+  void eucNodeDissipation();
+  int eucDeltat();
+  V2 eucNodeBC(int bc, const V2& bcv, const M22& Mp, const V2& Gp) const;
+  V2 eucNodeBCCorner(int bc1, const V2& bcv1, int bc2, const V2& bcv2, const M22& Mp, const V2& Gp) const;
+  void eucNodeVelocities();
+  void eucFaceVelocity();
+  void eucLagrangePosition();
+  void eucSubCellForce();
+  void eucLagrangeGeometry();
+  void eucFaceDeltax();
+  void eucCellUpdate();
+  int eucRemapFinal();
+  int eucIterate(bool dump, bool* continueLoop);
+  void eucDump();
+  void eucSwap();
 
   // remap helpers (remap/UtilesRemap.cc, remap/UtilesRemap-Impl.h)
   double fluxLimiter(int lim, double r) const;
@@ -385,6 +445,7 @@ void ref_ctx::allocate() {
   zfill(VerticalFaceOfNode, N), zfill(HorizontalFaceOfNode, N);
   zfill(TopupwindVelocity, N), zfill(BottomupwindVelocity, N), zfill(RightupwindVelocity, N);
   zfill(LeftupwindVelocity, N);
+  if (euc()) eucAllocate();
   // node coordinates: CartesianMesh2DGenerator.cc:52-55 (x = dx*i, y = dy*j)
   for (long j = 0; j <= mesh.ny; ++j)
     for (long i = 0; i <= mesh.nx; ++i) {
@@ -773,6 +834,7 @@ void ref_ctx::computeNodeMass() {
 
 // vnr_remap/VnrRemap.cc:330-341
 int ref_ctx::prologue() {
+  if (euc()) return eucPrologue();
   initBoundaryConditions();
   initCellPos();
   int st = init();
@@ -1947,21 +2009,22 @@ int ref_ctx::iterate(bool dump, bool* continueLoop) {
   return MHY_OK;
 }
 
-// cppnabla/mesh/PvdFileWriter2D.cc:26-114 via Vnr::dumpVariables
-// (vnr_remap/VnrRemap.cc:255-287)
-void ref_ctx::dumpVariables() {
-  nbCalls++;
-  if (writerDisabled || !(t_n >= lastDump + o.output_time || t_n == 0.)) return;
-  const std::string fileName = "VnrRemap." + std::to_string(nbCalls) + ".vtp";
+// cppnabla/mesh/PvdFileWriter2D.cc:26-114: one VTP file of module
+// <module> and its PVD collection (the point / cell data blocks only when
+// they hold fields)
+void ref_ctx::writeVtp(const std::string& module, long N, const V2* nodes, long C,
+                       const std::map<std::string, const double*>& cellVars,
+                       const std::map<std::string, const double*>& nodeVars,
+                       std::map<double, std::string>& byTimes) {
+  const std::string fileName = module + "." + std::to_string(nbCalls) + ".vtp";
   std::ofstream w(outdir + "/" + fileName);
-  const long N = mesh.nbNodes, C = mesh.nbCells;
   w << "<?xml version=\"1.0\"?>" << std::endl;
   w << "<VTKFile type=\"PolyData\">" << std::endl;
   w << "\t<PolyData>" << std::endl;
   w << "\t\t<Piece NumberOfPoints=\"" << N << "\" NumberOfPolys=\"" << C << "\">" << std::endl;
   w << "\t\t\t<Points>" << std::endl;
   w << "\t\t\t\t<DataArray type=\"Float32\" NumberOfComponents=\"3\" format=\"ascii\">" << std::endl;
-  for (long r = 0; r < N; ++r) w << " " << node_coord_n[r][0] << " " << node_coord_n[r][1] << " " << 0.0;
+  for (long r = 0; r < N; ++r) w << " " << nodes[r][0] << " " << nodes[r][1] << " " << 0.0;
   w << std::endl;
   w << "\t\t\t\t</DataArray>" << std::endl;
   w << "\t\t\t</Points>" << std::endl;
@@ -1978,6 +2041,39 @@ void ref_ctx::dumpVariables() {
   w << std::endl;
   w << "\t\t\t\t</DataArray>" << std::endl;
   w << "\t\t\t</Polys>" << std::endl;
+  auto block = [&](const char* tag, const std::map<std::string, const double*>& vars, long n) {
+    if (vars.empty()) return;
+    w << "\t\t\t<" << tag << " Scalars=\"" << vars.begin()->first << "\">" << std::endl;
+    for (auto& kv : vars) {
+      w << "\t\t\t\t<DataArray Name=\"" << kv.first << "\" type=\"Float32\" format=\"ascii\">" << std::endl;
+      for (long r = 0; r < n; ++r) w << " " << kv.second[r];
+      w << std::endl;
+      w << "\t\t\t\t</DataArray>" << std::endl;
+    }
+    w << "\t\t\t</" << tag << ">" << std::endl;
+  };
+  block("PointData", nodeVars, N);
+  block("CellData", cellVars, C);
+  w << "\t\t</Piece>" << std::endl;
+  w << "\t</PolyData>" << std::endl;
+  w << "</VTKFile>" << std::endl;
+  w.close();
+  byTimes[t_n] = fileName;
+  std::ofstream pvd(outdir + "/" + module + ".pvd");
+  pvd << "<?xml version=\"1.0\"?>" << std::endl;
+  pvd << "<VTKFile type=\"Collection\" version=\"0.1\">" << std::endl;
+  pvd << "\t<Collection>" << std::endl;
+  for (auto& kv : byTimes)
+    pvd << "\t\t\t<DataSet timestep=\"" << kv.first << "\" group=\"\" part=\"0\" file=\"" << kv.second << "\"/>"
+        << std::endl;
+  pvd << "\t</Collection>" << std::endl;
+  pvd << "</VTKFile>" << std::endl;
+}
+
+// Vnr::dumpVariables (vnr_remap/VnrRemap.cc:255-287)
+void ref_ctx::dumpVariables() {
+  nbCalls++;
+  if (writerDisabled || !(t_n >= lastDump + o.output_time || t_n == 0.)) return;
   std::map<std::string, const double*> nodeVars, cellVars;
   nodeVars["VitesseX"] = x_velocity.data();
   nodeVars["VitesseY"] = y_velocity.data();
@@ -1986,36 +2082,793 @@ void ref_ctx::dumpVariables() {
   cellVars["Energy"] = energy_n.data();
   if (o.nbmat > 1 && o.avec_projection == 1) cellVars["fracvol1"] = fracvol_env1.data();
   if (o.nbmat > 2 && o.avec_projection == 1) cellVars["fracvol2"] = fracvol_env2.data();
-  w << "\t\t\t<PointData Scalars=\"" << nodeVars.begin()->first << "\">" << std::endl;
-  for (auto& kv : nodeVars) {
-    w << "\t\t\t\t<DataArray Name=\"" << kv.first << "\" type=\"Float32\" format=\"ascii\">" << std::endl;
-    for (long r = 0; r < N; ++r) w << " " << kv.second[r];
-    w << std::endl;
-    w << "\t\t\t\t</DataArray>" << std::endl;
+  writeVtp("VnrRemap", mesh.nbNodes, node_coord_n.data(), mesh.nbCells, cellVars, nodeVars, fileNameByTimes);
+  lastDump = t_n;
+}
+
+// ================================================================ Eucclhyd
+//
+// The cell-centred Eucclhyd Lagrange scheme (eucclhyd_remap/), which runs
+// the same cell sweeps as VNR (not the dual remap) on a remap vector that
+// carries the cell momentum in components 3m, 3m+1 (and the kinetic energy
+// in 3m+2 when the projection is conservative).
+
+namespace {
+typedef std::array<double, 4> M22;
+// types/MathFunctions.h:41-47 (each row a dot product from 0.0)
+inline V2 matVec(const M22& a, const V2& b) { return {dot2({a[0], a[1]}, b), dot2({a[2], a[3]}, b)}; }
+// eucclhyd_remap/Utiles.cc:6-10 (types/MathFunctions.h:56-59 for det)
+inline M22 inverse2(const M22& a) {
+  const double alpha = 1.0 / (a[0] * a[3] - a[1] * a[2]);
+  return {a[3] * alpha, -a[1] * alpha, -a[2] * alpha, a[0] * alpha};
+}
+// eucclhyd_remap/Utiles-Impl.h:4-12
+inline M22 tens(const V2& a, const V2& b) { return {a[0] * b[0], a[0] * b[1], a[1] * b[0], a[1] * b[1]}; }
+inline M22 madd(const M22& a, const M22& b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2], a[3] + b[3]}; }
+inline M22 mscale(double s, const M22& a) { return {s * a[0], s * a[1], s * a[2], s * a[3]}; }
+constexpr double kEucCfl = 0.45;  // includes/GestionTemps.h:11 (never read from input)
+}  // namespace
+
+void ref_ctx::eucAllocate() {
+  const size_t C = mesh.nbCells, N = mesh.nbNodes, F = mesh.nbFaces;
+  zfill(e_lpc, C), zfill(e_nplus, C), zfill(e_nminus, C), zfill(e_lplus, C), zfill(e_lminus, C);
+  zfill(e_perimeter, C), zfill(e_cell_deltat, C), zfill(x_cell_velocity, C), zfill(y_cell_velocity, C);
+  zfill(cell_velocity_n, C), zfill(cell_velocity_nplus1, C), zfill(cell_velocity_n0, C);
+  zfill(e_velgrad, C), zfill(e_pgrad_env, C), zfill(e_p_extrap, C), zfill(e_pe_extrap, C), zfill(e_v_extrap, C);
+  zfill(e_F_n, C), zfill(e_F_nplus1, C), zfill(e_Fenv_n, C), zfill(e_Fenv_nplus1, C);
+  zfill(e_M, C), zfill(e_Menv, C), zfill(e_nodeM, N), zfill(e_nodeG, N), zfill(XfLagrange, F);
+  zfill(lagrange_volume, C);
+}
+
+// eucclhyd_remap/Init.cc:15-100 (a case without a rule keeps the
+// includes/ConditionsLimites.h:22-35 defaults, code 0: zero velocity)
+void ref_ctx::eucInitBoundaryConditions() {
+  const int sym = MHY_BC_SYMMETRY, imp = MHY_BC_IMPOSED_VELOCITY;
+  const V2 ex = {1.0, 0.0}, ey = {0.0, 1.0}, zero = {0.0, 0.0};
+  const int t = o.test_case;
+  if (t == MHY_CASE_SOD_X || t == MHY_CASE_SOD_Y || t == MHY_CASE_BI_SOD_X || t == MHY_CASE_BI_SOD_Y ||
+      t == MHY_CASE_TRIPLE_POINT || t == MHY_CASE_BI_TRIPLE_POINT) {
+    leftBC = sym, leftBCValue = ey, rightBC = sym, rightBCValue = ey;
+    topBC = sym, topBCValue = ex, bottomBC = sym, bottomBCValue = ex;
+  } else if (t == MHY_CASE_BI_SHOCK_BUBBLE) {
+    leftBC = sym, leftBCValue = ey;
+    rightBC = imp, rightBCValue = {-124.824, 0.0};
+    rightFluxBC = 1;
+    rightFluxBCValue = {1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, -124.824, 0.0, 250000, 0.0};
+    topBC = sym, topBCValue = ex, bottomBC = sym, bottomBCValue = ex;
+  } else if (t == MHY_CASE_SEDOV || t == MHY_CASE_BI_SEDOV || t == MHY_CASE_NOH || t == MHY_CASE_BI_NOH) {
+    leftBC = sym, leftBCValue = ey;
+    rightBC = imp, rightBCValue = zero;
+    topBC = imp, topBCValue = zero;
+    bottomBC = sym, bottomBCValue = ex;
   }
-  w << "\t\t\t</PointData>" << std::endl;
-  w << "\t\t\t<CellData Scalars=\"" << cellVars.begin()->first << "\">" << std::endl;
-  for (auto& kv : cellVars) {
-    w << "\t\t\t\t<DataArray Name=\"" << kv.first << "\" type=\"Float32\" format=\"ascii\">" << std::endl;
-    for (long c = 0; c < C; ++c) w << " " << kv.second[c];
-    w << std::endl;
-    w << "\t\t\t\t</DataArray>" << std::endl;
+  FluxBC = leftFluxBC + rightFluxBC + bottomFluxBC + topFluxBC;
+}
+
+// computeCornerNormal, eucclhyd_remap/PhaseLagrange.cc:27-66 (on the
+// set-up mesh; the time loop never forms them again)
+void ref_ctx::eucCornerNormals() {
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    auto nd = mesh.nodesOfCell(c);
+    for (int s = 0; s < 4; ++s) {
+      const V2& xp = node_coord_n[nd[s]];
+      const V2& xpPlus = node_coord_n[nd[(s + 1) % 4]];
+      const V2& xpMinus = node_coord_n[nd[(s + 3) % 4]];
+      V2 np = {0.5 * (xpPlus[1] - xp[1]), 0.5 * (xp[0] - xpPlus[0])};
+      const double lp = norm2(np);
+      np = vdiv(np, lp);
+      V2 nm = {0.5 * (xp[1] - xpMinus[1]), 0.5 * (xpMinus[0] - xp[0])};
+      const double lm = norm2(nm);
+      nm = vdiv(nm, lm);
+      e_nplus[c][s] = np, e_lplus[c][s] = lp;
+      e_nminus[c][s] = nm, e_lminus[c][s] = lm;
+      e_lpc[c][s] = vadd(vmul(np, lp), vmul(nm, lm));
+    }
   }
-  w << "\t\t\t</CellData>" << std::endl;
-  w << "\t\t</Piece>" << std::endl;
-  w << "\t</PolyData>" << std::endl;
-  w << "</VTKFile>" << std::endl;
-  w.close();
-  fileNameByTimes[t_n] = fileName;
-  std::ofstream pvd(outdir + "/VnrRemap.pvd");
-  pvd << "<?xml version=\"1.0\"?>" << std::endl;
-  pvd << "<VTKFile type=\"Collection\" version=\"0.1\">" << std::endl;
-  pvd << "\t<Collection>" << std::endl;
-  for (auto& kv : fileNameByTimes)
-    pvd << "\t\t\t<DataSet timestep=\"" << kv.first << "\" group=\"\" part=\"0\" file=\"" << kv.second << "\"/>"
-        << std::endl;
-  pvd << "\t</Collection>" << std::endl;
-  pvd << "</VTKFile>" << std::endl;
+}
+
+// eucclhyd_remap/Init.cc:106-167: polygon volume, centroid and perimeter
+void ref_ctx::eucInitMeshGeometryForCells() {
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    auto nd = mesh.nodesOfCell(c);
+    double r11 = 0.0;
+    for (int s = 0; s < 4; ++s) r11 = r11 + cross2(node_coord_n[nd[s]], node_coord_n[nd[(s + 1) % 4]]);
+    const double vol = 0.5 * r11;
+    V2 r12 = {0.0, 0.0};
+    for (int s = 0; s < 4; ++s) {
+      const V2& a = node_coord_n[nd[s]];
+      const V2& b = node_coord_n[nd[(s + 1) % 4]];
+      r12 = vadd(r12, vmul(vadd(a, b), cross2(a, b)));
+    }
+    cell_coord_n[c] = vmul(r12, 1.0 / (6.0 * vol));
+    euler_volume[c] = vol;
+    double r13 = 0.0;
+    for (int s = 0; s < 4; ++s) r13 = r13 + norm2(vsub(node_coord_n[nd[s]], node_coord_n[nd[(s + 1) % 4]]));
+    e_perimeter[c] = r13;
+  }
+}
+
+// initCellInternalEnergy, initCellVelocity, initDensity
+// (eucclhyd_remap/Init.cc:194-671).  Cases whose reference initialisation
+// leaves the density unset (Noh, BiNoh, BiSedov) are refused.
+int ref_ctx::eucInitState() {
+  const long C = mesh.nbCells;
+  const int t = o.test_case;
+  switch (t) {
+    case MHY_CASE_UNIT_TEST: case MHY_CASE_BI_UNIT_TEST: case MHY_CASE_SOD_X: case MHY_CASE_SOD_Y:
+    case MHY_CASE_BI_SOD_X: case MHY_CASE_BI_SOD_Y: case MHY_CASE_BI_SHOCK_BUBBLE: case MHY_CASE_SEDOV:
+    case MHY_CASE_TRIPLE_POINT: case MHY_CASE_BI_TRIPLE_POINT:
+      break;
+    default:
+      return MHY_ERR_UNKNOWN_CASE;
+  }
+  const double* g = o.gammap;
+  const V2 Xb = {0.320, 0.04};
+  const double rb = 0.025;
+  auto bubbleR = [&](const V2& x) { return std::sqrt((x[0] - Xb[0]) * (x[0] - Xb[0]) + (x[1] - Xb[1]) * (x[1] - Xb[1])); };
+  for (long c = 0; c < C; ++c) {
+    const V2& xc = cell_coord_n[c];
+    VM& ee = energy_env_n0[c];
+    ee = {0.0, 0.0, 0.0};
+    double& e = energy_n0[c];
+    // internal energy (Init.cc:194-385)
+    if (t == MHY_CASE_UNIT_TEST || t == MHY_CASE_BI_UNIT_TEST) {
+      const double p0 = 1., rho0 = 1., gamma = 1.4;
+      const double e0 = p0 / ((gamma - 1.0) * rho0);
+      e = e0;
+      for (int k = 0; k < NMAT; ++k) ee[k] = e0;
+    } else if (t == MHY_CASE_SOD_X || t == MHY_CASE_SOD_Y) {
+      const double r = (t == MHY_CASE_SOD_X) ? xc[0] : xc[1];
+      double pI, rI;
+      if (r <= 0.5) pI = 1.0, rI = 1.0;
+      else pI = 0.1, rI = 0.125;
+      const double eI = pI / ((g[0] - 1.0) * rI);
+      e = eI, ee[0] = eI;
+    } else if (t == MHY_CASE_BI_SOD_X || t == MHY_CASE_BI_SOD_Y) {
+      const double r = (t == MHY_CASE_BI_SOD_X) ? xc[0] : xc[1];
+      if (r <= 0.5) {
+        const double eI = 1.0 / ((g[0] - 1.0) * 1.0);
+        ee[0] = eI, ee[1] = 0., e = eI;
+      } else {
+        const double eI = 0.1 / ((g[1] - 1.0) * 0.125);
+        ee[0] = 0., ee[1] = eI, e = eI;
+      }
+    } else if (t == MHY_CASE_BI_SHOCK_BUBBLE) {
+      double eI = 1.e5 / ((g[0] - 1.0) * 1.0);
+      ee[0] = eI, ee[1] = 0., e = eI;
+      if (bubbleR(xc) <= rb) {
+        eI = 1.e5 / ((g[0] - 1.0) * 0.182);
+        ee[0] = 0., ee[1] = eI, e = eI;
+      }
+    } else if (t == MHY_CASE_SEDOV) {
+      bool center = false;
+      const double pI = 1.e-6, rI = 1.;
+      const double rmin = o.threshold;
+      const double e1 = pI / ((g[0] - 1.0) * rI);
+      for (long p : mesh.nodesOfCell(c))
+        if (norm2(node_coord_n[p]) < rmin) center = true;
+      if (center) e = e1 + 0.244816 / (o.dx * o.dy);
+      else e = e1;
+      ee[0] = e;
+    } else if (t == MHY_CASE_TRIPLE_POINT) {
+      double pI, rI;
+      if (xc[0] <= 0.01) pI = 1.0, rI = 1.0;
+      else if (xc[1] <= 0.015) pI = 0.1, rI = 1.;
+      else pI = 0.1, rI = 0.1;
+      const double eI = pI / ((g[0] - 1.0) * rI);
+      e = eI, ee[0] = eI;
+    } else {  // BiTriplePoint
+      if (xc[0] <= 0.01) {
+        const double eI = 1.0 / ((g[0] - 1.0) * 1.0);
+        e = eI, ee[0] = eI;
+      } else if (xc[1] <= 0.015) {
+        const double eI = 0.1 / ((g[1] - 1.0) * 1.);
+        e = eI, ee[1] = eI;
+      } else {
+        const double eI = 0.1 / ((g[2] - 1.0) * 0.1);
+        e = eI, ee[2] = eI;
+      }
+    }
+    // cell velocity (Init.cc:391-423); zero unless set
+    V2& v = cell_velocity_n0[c];
+    v = {0.0, 0.0};
+    if (t == MHY_CASE_BI_SHOCK_BUBBLE && xc[0] >= 0.60) v = {-124.824, 0.0};
+    if (t == MHY_CASE_UNIT_TEST || t == MHY_CASE_BI_UNIT_TEST) v = {1.0, 0.0};
+    // density, fractions (Init.cc:429-671)
+    VM& f = fracvol_env[c];
+    VM& mf = mass_fraction_env[c];
+    VM& re = density_env_n0[c];
+    double& rho = density_n0[c];
+    f = {0.0, 0.0, 0.0}, mf = {0.0, 0.0, 0.0}, re = {0.0, 0.0, 0.0};
+    if (t == MHY_CASE_UNIT_TEST || (t == MHY_CASE_BI_UNIT_TEST && xc[0] <= 0.5)) {
+      f[0] = 1., mf[0] = 1., rho = 1.0, re[0] = 1.0;
+    } else if (t == MHY_CASE_BI_UNIT_TEST) {
+      f[1] = 1., mf[1] = 1., rho = 1., re[1] = 1.0;
+    } else if (t == MHY_CASE_SOD_X || t == MHY_CASE_SOD_Y) {
+      const double r = (t == MHY_CASE_SOD_X) ? xc[0] : xc[1];
+      f[0] = 1., mf[0] = 1.;
+      const double d = (r <= 0.5) ? 1.0 : 0.125;
+      rho = d, re[0] = d, re[1] = d;
+    } else if (t == MHY_CASE_BI_SOD_X || t == MHY_CASE_BI_SOD_Y) {
+      const double r = (t == MHY_CASE_BI_SOD_X) ? xc[0] : xc[1];
+      if (r <= 0.5) f[0] = 1., mf[0] = 1., rho = 1.0, re[0] = 1.0;
+      else f[1] = 1., mf[1] = 1., rho = 0.125, re[1] = 0.125;
+    } else if (t == MHY_CASE_BI_SHOCK_BUBBLE) {
+      re[0] = 1.0, rho = 1.0, f[0] = 1., mf[0] = 1.;
+      if (bubbleR(xc) <= rb) {
+        re[0] = 0.0, re[1] = 0.182, rho = 0.182;
+        f[0] = 0., f[1] = 1., mf[0] = 0., mf[1] = 1.;
+      }
+    } else if (t == MHY_CASE_TRIPLE_POINT) {
+      f[0] = 1., mf[0] = 1.;  // the material density stays unset (zero), as in the reference
+      rho = (xc[0] <= 0.01) ? 1.0 : ((xc[1] <= 0.015) ? 1.0 : 0.1);
+    } else if (t == MHY_CASE_BI_TRIPLE_POINT) {
+      if (xc[0] <= 0.01) rho = 1.0, f[0] = 1., mf[0] = 1., re[0] = 1.0;
+      else if (xc[1] <= 0.015) rho = 1.0, f[1] = 1., mf[1] = 1., re[1] = 1.0;
+      else rho = 0.1, f[2] = 1., mf[2] = 1., re[2] = 0.1;
+    } else {  // Sedov
+      rho = 1.0, re[0] = 1.0;
+    }
+    if (o.nbmat == 1) f = {1., 0., 0.}, mf = {1., 0., 0.};
+    fracvol_env1[c] = f[0], fracvol_env2[c] = f[1], fracvol_env3[c] = f[2];
+    x_cell_velocity[c] = v[0], y_cell_velocity[c] = v[1];
+    int matcell = 0, imatpure = -1;
+    for (int k = 0; k < NMAT; ++k)
+      if (f[k] > o.threshold) matcell++, imatpure = k;
+    if (matcell > 1) mixte[c] = 1, pure[c] = -1;
+    else mixte[c] = 0, pure[c] = imatpure;
+  }
+  return MHY_OK;
+}
+
+// eucclhyd_remap/Init.cc:721-760 (t = 0 state and the case's initial time
+// step; GestTemps is value-initialised, so a case without one starts at 0)
+void ref_ctx::eucSetUpTimeLoopN() {
+  node_velocity_n = node_velocity_n0;
+  density_n = density_n0, density_env_n = density_env_n0;
+  cell_velocity_n = cell_velocity_n0;
+  energy_n = energy_n0, energy_env_n = energy_env_n0;
+  const int t = o.test_case;
+  deltat_n = 0.0;
+  if (t == MHY_CASE_SOD_X || t == MHY_CASE_SOD_Y || t == MHY_CASE_BI_SOD_X || t == MHY_CASE_BI_SOD_Y ||
+      t == MHY_CASE_SEDOV)
+    deltat_n = 1.0e-4;
+  else if (t == MHY_CASE_BI_SHOCK_BUBBLE)
+    deltat_n = 1.0e-7;
+  else if (t == MHY_CASE_TRIPLE_POINT || t == MHY_CASE_BI_TRIPLE_POINT)
+    deltat_n = 1.0e-5;
+  t_n = 0.0;
+}
+
+// Eucclhyd::simulate set-up (eucclhyd_remap/EucclhydRemap.cc:273-286)
+int ref_ctx::eucPrologue() {
+  for (int k = 0; k < o.nbmat; ++k)
+    if (o.eos[k] != MHY_EOS_PERFECT_GAS && o.eos[k] != MHY_EOS_VOID) return MHY_ERR_ARG;
+  if (o.space_order != 1 && o.space_order != 2) return MHY_ERR_ARG;
+  node_coord_n = node_coord_n0;
+  eucCornerNormals();
+  eucInitMeshGeometryForCells();
+  eucInitBoundaryConditions();
+  int st = eucInitState();
+  if (st != MHY_OK) return st;
+  cell_coord_n0 = cell_coord_n;
+  initMeshGeometryForFaces();  // same rule as vnr_remap/Init.cc:517-554 (Init.cc:677-715)
+  eucSetUpTimeLoopN();
+  return MHY_OK;
+}
+
+// computeEOS (PhaseLagrange.cc:70-105): perfect gas and void
+void ref_ctx::eucComputeEOS() {
+  for (int k = 0; k < o.nbmat; ++k) {
+    const double g = o.gammap[k];
+    for (long c = 0; c < mesh.nbCells; ++c) {
+      if (o.eos[k] == MHY_EOS_PERFECT_GAS) {
+        pressure_env_n[c][k] = (g - 1.0) * density_env_n[c][k] * energy_env_n[c][k];
+        speed_env_n[c][k] = (density_env_n[c][k] > 0.) ? std::sqrt(g * pressure_env_n[c][k] / density_env_n[c][k]) : 1.e-20;
+      } else {
+        pressure_env_n[c][k] = 0.;
+        speed_env_n[c][k] = 1.e-20;
+      }
+    }
+  }
+}
+
+// computePressionMoyenne (PhaseLagrange.cc:139-154): the sound speed keeps
+// its previous value in the max unless the density is positive
+void ref_ctx::eucPressionMoyenne() {
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    pressure_n[c] = 0.;
+    for (int k = 0; k < o.nbmat; ++k) {
+      pressure_n[c] += fracvol_env[c][k] * pressure_env_n[c][k];
+      speed_n[c] = smax(speed_n[c], speed_env_n[c][k]);
+    }
+    if (density_n[c] > 0.) speed_n[c] = std::sqrt(o.gammap[0] * pressure_n[c] / density_n[c]);
+  }
+}
+
+// computeGradients, second-order part (PhaseLagrange.cc:199-244): velocity
+// gradient from the node velocities at n, material pressure gradients from
+// the corner forces at n (the particle-only reductions are not formed)
+void ref_ctx::eucGradients() {
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    auto nd = mesh.nodesOfCell(c);
+    M22 r = {0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < 4; ++s) r = madd(r, tens(node_velocity_n[nd[s]], e_lpc[c][s]));
+    const double V = euler_volume[c];
+    e_velgrad[c] = {r[0] / V, r[1] / V, r[2] / V, r[3] / V};
+    for (int k = 0; k < NMAT; ++k) {
+      V2 gk = {0.0, 0.0};
+      for (int s = 0; s < 4; ++s) gk = vadd(gk, e_Fenv_n[c][s][k]);
+      e_pgrad_env[c][k] = vdiv(gk, V);
+    }
+  }
+}
+
+// computeMass (PhaseLagrange.cc:251-260)
+void ref_ctx::eucMass() {
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    cell_mass[c] = density_n[c] * euler_volume[c];
+    for (int k = 0; k < o.nbmat; ++k) cell_mass_env[c][k] = mass_fraction_env[c][k] * cell_mass[c];
+  }
+}
+
+// computeDissipationMatrix (PhaseLagrange.cc:266-296)
+void ref_ctx::eucDissipationMatrix() {
+  for (long c = 0; c < mesh.nbCells; ++c)
+    for (int s = 0; s < 4; ++s) {
+      const M22 cm = madd(mscale(e_lplus[c][s], tens(e_nplus[c][s], e_nplus[c][s])),
+                          mscale(e_lminus[c][s], tens(e_nminus[c][s], e_nminus[c][s])));
+      e_M[c][s] = mscale(density_n[c] * speed_n[c], cm);
+      for (int k = 0; k < NMAT; ++k) e_Menv[c][s][k] = mscale(density_env_n[c][k] * speed_n[c], cm);
+    }
+}
+
+// computem_cell_deltat (PhaseLagrange.cc:302-316)
+void ref_ctx::eucCellDeltat() {
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    if (o.avec_projection == 1)
+      e_cell_deltat[c] = euler_volume[c] / (e_perimeter[c] * (norm2(cell_velocity_n[c]) + speed_n[c]));
+    else
+      e_cell_deltat[c] = euler_volume[c] / (e_perimeter[c] * speed_n[c]);
+  }
+}
+
+// extrapolateValue (PhaseLagrange.cc:322-478): second order extrapolates
+// the material pressures and the velocity to each corner, clamped to the
+// range of the node's cells (max reductions start at DBL_MIN, the smallest
+// positive double, as the reference's numeric_limits<double>::min())
+void ref_ctx::eucExtrapolate() {
+  const int m = o.nbmat;
+  if (o.space_order == 1) {
+    for (long c = 0; c < mesh.nbCells; ++c)
+      for (int s = 0; s < 4; ++s) {
+        e_v_extrap[c][s] = cell_velocity_n[c];
+        e_p_extrap[c][s] = pressure_n[c];
+        for (int k = 0; k < m; ++k) e_pe_extrap[c][s][k] = pressure_env_n[c][k];
+      }
+    return;
+  }
+  const double big = std::numeric_limits<double>::max(), tiny = std::numeric_limits<double>::min();
+  for (long p = 0; p < mesh.nbNodes; ++p) {
+    long cells[4];
+    int slot[4];
+    const int nc = mesh.cellsOfNode(p, cells, slot);
+    double minP[NMAT], maxP[NMAT];
+    for (int k = 0; k < NMAT; ++k) minP[k] = big, maxP[k] = tiny;
+    for (int q = 0; q < nc; ++q)
+      for (int k = 0; k < NMAT; ++k) minP[k] = smin(minP[k], pressure_env_n[cells[q]][k]);
+    for (int q = 0; q < nc; ++q)
+      for (int k = 0; k < NMAT; ++k) maxP[k] = smax(maxP[k], pressure_env_n[cells[q]][k]);
+    double minVx = big, maxVx = tiny, minVy = big, maxVy = tiny;
+    for (int q = 0; q < nc; ++q) minVx = smin(minVx, cell_velocity_n[cells[q]][0]);
+    for (int q = 0; q < nc; ++q) maxVx = smax(maxVx, cell_velocity_n[cells[q]][0]);
+    for (int q = 0; q < nc; ++q) minVy = smin(minVy, cell_velocity_n[cells[q]][1]);
+    for (int q = 0; q < nc; ++q) maxVy = smax(maxVy, cell_velocity_n[cells[q]][1]);
+    for (int q = 0; q < nc; ++q) {
+      const long c = cells[q];
+      const int s = slot[q];
+      const V2 d = vsub(node_coord_n[p], cell_coord_n[c]);
+      for (int k = 0; k < NMAT; ++k) {
+        const double ptmp = pressure_env_n[c][k] + dot2(e_pgrad_env[c][k], d);
+        e_pe_extrap[c][s][k] = smax(smin(maxP[k], ptmp), minP[k]);
+      }
+      e_p_extrap[c][s] = 0.;
+      for (int k = 0; k < m; ++k) e_p_extrap[c][s] += fracvol_env[c][k] * e_pe_extrap[c][s][k];
+      const V2 vt = vadd(cell_velocity_n[c], matVec(e_velgrad[c], d));
+      e_v_extrap[c][s] = {smax(smin(maxVx, vt[0]), minVx), smax(smin(maxVy, vt[1]), minVy)};
+    }
+  }
+}
+
+// computeG (PhaseLagrange.cc:484-504)
+void ref_ctx::eucNodeG() {
+  for (long p = 0; p < mesh.nbNodes; ++p) {
+    long cells[4];
+    int slot[4];
+    const int nc = mesh.cellsOfNode(p, cells, slot);
+    V2 r = {0.0, 0.0};
+    for (int q = 0; q < nc; ++q) {
+      const long c = cells[q];
+      const int s = slot[q];
+      r = vadd(r, vadd(matVec(e_M[c][s], e_v_extrap[c][s]), vmul(e_lpc[c][s], e_p_extrap[c][s])));
+    }
+    e_nodeG[p] = r;
+  }
+}
+
+// computeNodeDissipationMatrixAndG (PhaseLagrange.cc:511-526)
+void ref_ctx::eucNodeDissipation() {
+  for (long p = 0; p < mesh.nbNodes; ++p) {
+    long cells[4];
+    int slot[4];
+    const int nc = mesh.cellsOfNode(p, cells, slot);
+    M22 r = {0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < nc; ++q) r = madd(r, e_M[cells[q]][slot[q]]);
+    e_nodeM[p] = r;
+  }
+}
+
+// computedeltat (EucclhydRemap.cc:203-227): Kokkos::Min keeps the smaller
+// value (a NaN never replaces it)
+int ref_ctx::eucDeltat() {
+  double red = std::numeric_limits<double>::max();
+  for (long c = 0; c < mesh.nbCells; ++c)
+    if (e_cell_deltat[c] < red) red = e_cell_deltat[c];
+  deltat_nplus1 = smin(kEucCfl * red, deltat_n * 1.05);
+  if (deltat_nplus1 < o.deltat_min) return MHY_ERR_DT_MIN;
+  return MHY_OK;
+}
+
+// nodeVelocityBoundaryCondition (ConditionsLimites.cc:109-123)
+V2 ref_ctx::eucNodeBC(int bc, const V2& bcv, const M22& Mp, const V2& Gp) const {
+  if (bc == MHY_BC_SYMMETRY) return vmul(bcv, dot2(Gp, bcv) / dot2(matVec(Mp, bcv), bcv));
+  if (bc == MHY_BC_IMPOSED_VELOCITY) return bcv;
+  if (bc == MHY_BC_FREE_SURFACE) return matVec(inverse2(Mp), Gp);
+  return {0.0, 0.0};
+}
+
+// nodeVelocityBoundaryConditionCorner (ConditionsLimites.cc:125-145)
+V2 ref_ctx::eucNodeBCCorner(int bc1, const V2& bcv1, int bc2, const V2& bcv2, const M22& Mp, const V2& Gp) const {
+  if (bc1 == MHY_BC_SYMMETRY && bc2 == MHY_BC_SYMMETRY) {
+    if (std::fabs(std::fabs(dot2(bcv1, bcv2)) - norm2(bcv1) * norm2(bcv2)) < 1.0E-8)
+      return vmul(bcv1, dot2(Gp, bcv1) / dot2(matVec(Mp, bcv1), bcv1));
+    return {0.0, 0.0};
+  }
+  if (bc1 == MHY_BC_IMPOSED_VELOCITY && bc2 == MHY_BC_IMPOSED_VELOCITY) return vmul(vadd(bcv1, bcv2), 0.5);
+  if (bc1 == MHY_BC_FREE_SURFACE && bc2 == MHY_BC_FREE_SURFACE) return matVec(inverse2(Mp), Gp);
+  return {0.0, 0.0};
+}
+
+// computeBoundaryNodeVelocities (ConditionsLimites.cc:19-107: sides left,
+// right, top, bottom with their corners, then the four corners) and
+// computeNodeVelocity (PhaseLagrange.cc:532-542) for the inner nodes
+void ref_ctx::eucNodeVelocities() {
+  const long nx = mesh.nx, ny = mesh.ny;
+  auto& v = node_velocity_nplus1;
+  auto at = [&](long j, long i) { return mesh.node(j, i); };
+  for (long j = 0; j <= ny; ++j) v[at(j, 0)] = eucNodeBC(leftBC, leftBCValue, e_nodeM[at(j, 0)], e_nodeG[at(j, 0)]);
+  for (long j = 0; j <= ny; ++j)
+    v[at(j, nx)] = eucNodeBC(rightBC, rightBCValue, e_nodeM[at(j, nx)], e_nodeG[at(j, nx)]);
+  for (long i = 0; i <= nx; ++i) v[at(ny, i)] = eucNodeBC(topBC, topBCValue, e_nodeM[at(ny, i)], e_nodeG[at(ny, i)]);
+  for (long i = 0; i <= nx; ++i)
+    v[at(0, i)] = eucNodeBC(bottomBC, bottomBCValue, e_nodeM[at(0, i)], e_nodeG[at(0, i)]);
+  long p = at(ny, 0);
+  v[p] = eucNodeBCCorner(topBC, topBCValue, leftBC, leftBCValue, e_nodeM[p], e_nodeG[p]);
+  p = at(ny, nx);
+  v[p] = eucNodeBCCorner(topBC, topBCValue, rightBC, rightBCValue, e_nodeM[p], e_nodeG[p]);
+  p = at(0, 0);
+  v[p] = eucNodeBCCorner(bottomBC, bottomBCValue, leftBC, leftBCValue, e_nodeM[p], e_nodeG[p]);
+  p = at(0, nx);
+  v[p] = eucNodeBCCorner(bottomBC, bottomBCValue, rightBC, rightBCValue, e_nodeM[p], e_nodeG[p]);
+  for (long q = 0; q < mesh.nbNodes; ++q)
+    if (mesh.innerNode(q)) v[q] = matVec(inverse2(e_nodeM[q]), e_nodeG[q]);
+}
+
+// computeFaceVelocity (PhaseLagrange.cc:548-566)
+void ref_ctx::eucFaceVelocity() {
+  for (long f = 0; f < mesh.nbFaces; ++f) {
+    V2 r = {0.0, 0.0};
+    r = vadd(r, node_velocity_nplus1[mesh.firstNode(f)]);
+    r = vadd(r, node_velocity_nplus1[mesh.secondNode(f)]);
+    faceNormalVelocity[f] = dot2(vmul(r, 0.5), faceNormal[f]);
+  }
+}
+
+// computeLagrangePosition (PhaseLagrange.cc:573-618): the nodes move with
+// the time step of step n; without remap the perimeter of the start-of-step
+// mesh is kept for the next step's time step
+void ref_ctx::eucLagrangePosition() {
+  for (long p = 0; p < mesh.nbNodes; ++p) XLagrange[p] = vadd(node_coord_n[p], vmul(node_velocity_nplus1[p], deltat_n));
+  for (long f = 0; f < mesh.nbFaces; ++f) {
+    const long n1 = mesh.firstNode(f), n2 = mesh.secondNode(f);
+    XfLagrange[f] = vmul(vadd(XLagrange[n1], XLagrange[n2]), 0.5);
+    faceLengthLagrange[f] = norm2(vsub(XLagrange[n2], XLagrange[n1]));
+  }
+  if (o.avec_projection == 0) {
+    for (long c = 0; c < mesh.nbCells; ++c) {
+      auto nd = mesh.nodesOfCell(c);
+      double r = 0.0;
+      for (int s = 0; s < 4; ++s) r = r + norm2(vsub(node_coord_n[nd[s]], node_coord_n[nd[(s + 1) % 4]]));
+      e_perimeter[c] = r;
+    }
+  }
+}
+
+// computeSubCellForce (PhaseLagrange.cc:625-666)
+void ref_ctx::eucSubCellForce() {
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    auto nd = mesh.nodesOfCell(c);
+    for (int s = 0; s < 4; ++s) {
+      const V2 dv = vsub(node_velocity_nplus1[nd[s]], e_v_extrap[c][s]);
+      const V2& l = e_lpc[c][s];
+      e_F_nplus1[c][s] = vadd(vmul(l, -e_p_extrap[c][s]), matVec(e_M[c][s], dv));
+      for (int k = 0; k < NMAT; ++k)
+        e_Fenv_nplus1[c][s][k] = vadd(vmul(l, -e_pe_extrap[c][s][k]), matVec(e_Menv[c][s][k], dv));
+    }
+  }
+}
+
+// computeLagrangeVolumeAndCenterOfGravity (PhaseLagrange.cc:671-715)
+void ref_ctx::eucLagrangeGeometry() {
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    auto nd = mesh.nodesOfCell(c);
+    double r6 = 0.0;
+    for (int s = 0; s < 4; ++s) r6 = r6 + cross2(XLagrange[nd[s]], XLagrange[nd[(s + 1) % 4]]);
+    const double vol = 0.5 * r6;
+    vLagrange[c] = vol;
+    for (int k = 0; k < o.nbmat; ++k) lagrange_volume[c][k] = fracvol_env[c][k] * vol;
+    V2 r7 = {0.0, 0.0};
+    for (int s = 0; s < 4; ++s) {
+      const V2& a = XLagrange[nd[s]];
+      const V2& b = XLagrange[nd[(s + 1) % 4]];
+      r7 = vadd(r7, vmul(vadd(a, b), cross2(a, b)));
+    }
+    XcLagrange[c] = vmul(r7, 1.0 / (6.0 * vol));
+  }
+}
+
+// computeFacedeltaxLagrange (PhaseLagrange.cc:721-739)
+void ref_ctx::eucFaceDeltax() {
+  for (long f = 0; f < mesh.nbFaces; ++f) {
+    if (!mesh.innerFace(f)) continue;
+    long b, fr;
+    mesh.cellsOfFace(f, b, fr);
+    deltaxLagrange[f] = dot2(vsub(XcLagrange[fr], XcLagrange[b]), faceNormal[f]);
+  }
+}
+
+// updateCellCenteredLagrangeVariables (PhaseLagrange.cc:746-1011)
+void ref_ctx::eucCellUpdate() {
+  const int m = o.nbmat;
+  const double dt = deltat_n, thr = o.threshold;
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    auto nd = mesh.nodesOfCell(c);
+    double r2 = 0.0;
+    for (int s = 0; s < 4; ++s) r2 = r2 + dot2(e_lpc[c][s], node_velocity_nplus1[nd[s]]);
+    const double rhoL = 1 / (1 / density_n[c] + dt / cell_mass[c] * r2);
+    V2 r3 = {0.0, 0.0};
+    for (int s = 0; s < 4; ++s) r3 = vadd(r3, e_F_nplus1[c][s]);
+    const V2 vc = cell_velocity_n[c];
+    const V2 vL = vadd(vc, vdiv(vmul(r3, dt), cell_mass[c]));
+    double r4 = 0.0;
+    VM pr4 = {0.0, 0.0, 0.0};
+    for (int s = 0; s < 4; ++s) {
+      const V2 w = vsub(node_velocity_nplus1[nd[s]], vmul(vadd(vc, vL), 0.5));
+      r4 = r4 + dot2(e_F_nplus1[c][s], w);
+      for (int k = 0; k < NMAT; ++k) pr4[k] = pr4[k] + dot2(e_Fenv_nplus1[c][s][k], w);
+    }
+    (void)r4;  // eLagrange is not used by the reference
+    VM peL = {0.0, 0.0, 0.0};
+    const VM& f = fracvol_env[c];
+    const VM& mf = mass_fraction_env[c];
+    for (int k = 0; k < m; ++k)
+      if (f[k] > thr && cell_mass_env[c][k] != 0.)
+        peL[k] = energy_env_n[c][k] + f[k] * dt / cell_mass_env[c][k] * pr4[k];
+    VE& U = ULagrange[c];
+    const double vol = vLagrange[c];
+    for (int k = 0; k < m; ++k) {
+      U[k] = lagrange_volume[c][k];
+      U[m + k] = mf[k] * vol * rhoL;
+      U[2 * m + k] = mf[k] * vol * rhoL * peL[k];
+    }
+    U[3 * m] = vol * rhoL * vL[0];
+    U[3 * m + 1] = vol * rhoL * vL[1];
+    if (o.projection_conservative == 1) U[3 * m + 2] = 0.5 * vol * rhoL * (vL[0] * vL[0] + vL[1] * vL[1]);
+    if (o.avec_projection == 0) {
+      cell_velocity_nplus1[c] = vL;
+      density_nplus1[c] = 0.;
+      energy_nplus1[c] = 0.;
+      for (int k = 0; k < m; ++k) {
+        density_nplus1[c] += mf[k] * rhoL;
+        density_env_nplus1[c][k] = (f[k] > thr) ? mf[k] * rhoL / f[k] : 0.;
+        energy_nplus1[c] += mf[k] * peL[k];
+        energy_env_nplus1[c][k] = (f[k] > thr) ? peL[k] : 0.;
+      }
+      fracvol_env1[c] = f[0], fracvol_env2[c] = f[1], fracvol_env3[c] = f[2];
+      if (vL[0] > 0.) x_cell_velocity[c] = smax(vL[0], thr);
+      if (vL[0] < 0.) x_cell_velocity[c] = smin(vL[0], -thr);
+      if (vL[1] > 0.) y_cell_velocity[c] = smax(vL[1], thr);
+      if (vL[1] < 0.) y_cell_velocity[c] = smin(vL[1], -thr);
+    }
+    VE& P = Phi[c];
+    if (o.pp == 1) {
+      double sv = 0.;
+      for (int k = 0; k < m; ++k) sv += U[k];
+      double sm = 0.;
+      for (int k = 0; k < m; ++k) {
+        P[k] = U[k] / sv;
+        P[m + k] = (U[k] != 0.) ? U[m + k] / U[k] : 0.;
+        sm += U[m + k];
+      }
+      P[3 * m] = U[3 * m] / sm;
+      P[3 * m + 1] = U[3 * m + 1] / sm;
+      for (int k = 0; k < m; ++k) P[2 * m + k] = (U[m + k] != 0.) ? U[2 * m + k] / U[m + k] : 0.;
+      if (o.projection_conservative == 1) P[3 * m + 2] = U[3 * m + 2] / sm;
+    } else {
+      for (int e = 0; e < NEQ; ++e) P[e] = U[e] / vol;
+    }
+  }
+}
+
+// remapCellcenteredVariable (eucclhyd_remap/PhaseRemapFinal.cc:17-232)
+int ref_ctx::eucRemapFinal() {
+  x_then_y_nplus1 = !x_then_y_n;
+  const int m = o.nbmat;
+  const double thr = o.threshold;
+  int bad = 0;
+  for (long c = 0; c < mesh.nbCells; ++c) {
+    const VE& U = Uremap2[c];
+    const double vol = euler_volume[c];
+    double volt = 0., masset = 0.;
+    VM vn = {0.0, 0.0, 0.0};
+    for (int k = 0; k < m; ++k) {
+      vn[k] = U[k];
+      volt += vn[k];
+      masset += U[m + k];
+    }
+    double vtn = 0.;
+    for (int k = 0; k < m; ++k) {
+      vn[k] *= vol / volt;
+      vtn += vn[k];
+    }
+    VM& f = fracvol_env[c];
+    double sf = 0.;
+    for (int k = 0; k < m; ++k) {
+      f[k] = vn[k] / vtn;
+      if (f[k] < thr) f[k] = 0.;
+      sf += f[k];
+    }
+    for (int k = 0; k < m; ++k) f[k] = f[k] / sf;
+    int matcell = 0, imatpure = -1;
+    for (int k = 0; k < m; ++k)
+      if (f[k] > 0.) matcell++, imatpure = k;
+    if (matcell > 1) mixte[c] = 1, pure[c] = -1;
+    else mixte[c] = 0, pure[c] = imatpure;
+    VM& mf = mass_fraction_env[c];
+    for (int k = 0; k < m; ++k) mf[k] = U[m + k] / masset;
+    double fm = 0.;
+    for (int k = 0; k < m; ++k) {
+      if (f[k] < thr) mf[k] = 0.;
+      fm += mf[k];
+    }
+    for (int k = 0; k < m; ++k) mf[k] /= fm;
+    VM dens = {0.0, 0.0, 0.0};
+    double dn = 0.;
+    for (int k = 0; k < m; ++k) {
+      if (f[k] > thr) dens[k] = U[m + k] / vn[k];
+      dn += f[k] * dens[k];
+    }
+    const V2 v = {U[3 * m] / (dn * vol), U[3 * m + 1] / (dn * vol)};
+    VM pe = {0.0, 0.0, 0.0};
+    for (int k = 0; k < m; ++k)
+      if ((f[k] > thr) && (U[m + k] != 0.)) pe[k] = U[2 * m + k] / U[m + k];
+    density_nplus1[c] = dn;
+    cell_velocity_nplus1[c] = v;
+    energy_nplus1[c] = 0.;
+    double dec = 0.;
+    if (o.projection_conservative == 1) dec = U[3 * m + 2] / masset - 0.5 * (v[0] * v[0] + v[1] * v[1]);
+    for (int k = 0; k < m; ++k) {
+      density_env_nplus1[c][k] = dens[k];
+      energy_env_nplus1[c][k] = pe[k];
+      energy_env_nplus1[c][k] += dec;
+      energy_nplus1[c] += mf[k] * energy_env_nplus1[c][k];
+    }
+    for (int k = 0; k < m; ++k) {
+      if (pe[k] < 0. || dens[k] < 0.) {
+        density_env_nplus1[c][k] = 0.;
+        energy_env_nplus1[c][k] = 0.;
+        mf[k] = 0.;
+        f[k] = 0.;
+      }
+    }
+    if (energy_nplus1[c] != energy_nplus1[c] || density_nplus1[c] != density_nplus1[c]) bad = 1;
+    fracvol_env1[c] = f[0], fracvol_env2[c] = f[1], fracvol_env3[c] = f[2];
+    if (v[0] > 0.) x_cell_velocity[c] = smax(v[0], thr);
+    if (v[0] < 0.) x_cell_velocity[c] = smin(v[0], -thr);
+    if (v[1] > 0.) y_cell_velocity[c] = smax(v[1], thr);
+    if (v[1] < 0.) y_cell_velocity[c] = smin(v[1], -thr);
+  }
+  return bad ? MHY_ERR_NAN : MHY_OK;
+}
+
+// Eucclhyd::executeTimeLoopN body (EucclhydRemap.cc:84-142)
+int ref_ctx::eucIterate(bool dump, bool* continueLoop) {
+  n++;
+  eucComputeEOS();
+  eucPressionMoyenne();
+  if (o.space_order == 2) eucGradients();
+  eucMass();
+  eucDissipationMatrix();
+  eucCellDeltat();
+  if (dump) eucDump();
+  eucExtrapolate();
+  eucNodeG();
+  eucNodeDissipation();
+  int st = eucDeltat();
+  if (st != MHY_OK) return st;
+  eucNodeVelocities();
+  t_nplus1 = t_n + deltat_nplus1;
+  eucFaceVelocity();
+  eucLagrangePosition();
+  eucSubCellForce();
+  eucLagrangeGeometry();
+  eucFaceDeltax();
+  eucCellUpdate();
+  if (o.avec_projection == 1) {
+    for (int s = 1; s <= 2; ++s) {
+      computeGradPhiFace(s);
+      computeGradPhi(s);
+      computeUpwindFaceQuantitiesForProjection(s);
+      computeUremap(s);
+    }
+    st = eucRemapFinal();
+    if (st != MHY_OK) return st;
+  }
+  *continueLoop = (n + 1 < o.max_time_iterations && t_nplus1 < o.final_time);
+  return MHY_OK;
+}
+
+// the n / n+1 swaps of EucclhydRemap.cc:143-168
+void ref_ctx::eucSwap() {
+  std::swap(x_then_y_nplus1, x_then_y_n);
+  std::swap(t_nplus1, t_n);
+  std::swap(deltat_nplus1, deltat_n);
+  std::swap(node_velocity_nplus1, node_velocity_n);
+  std::swap(density_nplus1, density_n);
+  std::swap(density_env_nplus1, density_env_n);
+  std::swap(cell_velocity_nplus1, cell_velocity_n);
+  std::swap(energy_nplus1, energy_n);
+  std::swap(energy_env_nplus1, energy_env_n);
+  std::swap(e_F_nplus1, e_F_n);
+  std::swap(e_Fenv_nplus1, e_Fenv_n);
+  if (o.avec_projection == 0) {
+    std::swap(vLagrange, euler_volume);
+    std::swap(XLagrange, node_coord_n);
+    std::swap(XfLagrange, Xf);
+    std::swap(faceLengthLagrange, faceLength);
+    std::swap(XcLagrange, cell_coord_n);
+  }
+}
+
+// dumpVariables (EucclhydRemap.cc:24-62): the cell fields, and the empty
+// particle file of the disabled particle scheme
+void ref_ctx::eucDump() {
+  nbCalls++;
+  if (writerDisabled || !(t_n >= lastDump + o.output_time || t_n == 0.)) return;
+  std::map<std::string, const double*> cellVars, nodeVars, partVars;
+  cellVars["Pressure"] = pressure_n.data();
+  cellVars["Density"] = density_n.data();
+  cellVars["F1"] = fracvol_env1.data();
+  cellVars["F2"] = fracvol_env2.data();
+  cellVars["F3"] = fracvol_env3.data();
+  cellVars["VelocityX"] = x_cell_velocity.data();
+  cellVars["VelocityY"] = y_cell_velocity.data();
+  cellVars["Energy"] = energy_n.data();
+  partVars["VolumePart"] = nullptr;
+  partVars["VxPart"] = nullptr;
+  partVars["VyPart"] = nullptr;
+  writeVtp("EucclhydRemap", mesh.nbNodes, node_coord_n.data(), mesh.nbCells, cellVars, nodeVars, fileNameByTimes);
+  writeVtp("Particules", 0, nullptr, 0, cellVars, partVars, particleFileNameByTimes);
   lastDump = t_n;
 }
 
@@ -2188,10 +3041,11 @@ int ref_probe_cell_pp(ref_ctx* r, const double* gp, const double* gm, const doub
 int ref_step(ref_ctx* r) {
   if (r->status != MHY_OK) return r->status;
   bool cont = true;
-  int st = r->iterate(false, &cont);
+  int st = r->euc() ? r->eucIterate(false, &cont) : r->iterate(false, &cont);
   if (st != MHY_OK) return r->status = st;
   r->finished = !cont;
-  r->swapAll();
+  if (r->euc()) r->eucSwap();
+  else r->swapAll();
   return MHY_OK;
 }
 
@@ -2205,12 +3059,15 @@ int ref_simulate(ref_ctx* r, const char* output_dir, int max_steps) {
   r->n = 0;
   bool cont = true;
   do {
-    int st = r->iterate(true, &cont);
+    int st = r->euc() ? r->eucIterate(true, &cont) : r->iterate(true, &cont);
     if (st != MHY_OK) return r->status = st;
     if (max_steps > 0 && r->n >= max_steps) cont = false;
-    if (cont) r->swapAll();
+    if (cont) {
+      if (r->euc()) r->eucSwap();
+      else r->swapAll();
+    }
   } while (cont);
-  r->dumpVariables();
+  if (!r->euc()) r->dumpVariables();  // Vnr::simulate dumps once more; Eucclhyd::simulate does not
   r->finished = true;
   return MHY_OK;
 }
@@ -2245,6 +3102,17 @@ int ref_get_field(ref_ctx* r, int field, double* out, long long n) {
     case MHY_CELL_SOUND_SPEED: return cellv(r->speed_n);
     case MHY_CELL_DIVU: return cellv(r->divu_n);
     case MHY_CELL_MASS: return cellv(r->cell_mass);
+    case MHY_CELL_VELOCITY_X:
+    case MHY_CELL_VELOCITY_Y:
+      if (!r->euc() || n != C) return MHY_ERR_ARG;
+      for (long c = 0; c < C; ++c) out[c] = r->cell_velocity_n[c][field == MHY_CELL_VELOCITY_X ? 0 : 1];
+      return MHY_OK;
+    case MHY_CELL_OUT_VX:
+      if (!r->euc()) return MHY_ERR_ARG;
+      return cellv(r->x_cell_velocity);
+    case MHY_CELL_OUT_VY:
+      if (!r->euc()) return MHY_ERR_ARG;
+      return cellv(r->y_cell_velocity);
     case MHY_CELL_MIXTE:
     case MHY_CELL_PURE:
       if (n != C) return MHY_ERR_ARG;

@@ -1,8 +1,8 @@
 /*
  * ORACLE -- test infrastructure, not product code.
  *
- * C API of the CPU restatement of MaHyCo's VNR Lagrange + directional remap
- * time loop (oracle/refcpu.cc).  Only tests/, __graft_entry__.smoke() and
+ * C API of the CPU restatement of MaHyCo's VNR and Eucclhyd Lagrange +
+ * directional remap time loops (oracle/refcpu.cc).  Only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg load it, as the checker.
  */
 #ifndef MAHYCO_ORACLE_REFCPU_H
@@ -20,11 +20,13 @@ typedef struct ref_ctx ref_ctx;
 int ref_read_donnees(const char* path, mhy_config* cfg);
 void ref_config_default(mhy_config* cfg);
 
-/* Mesh + initial state + setUpTimeLoopN (VnrRemap.cc:330-341). */
+/* Mesh + initial state + setUpTimeLoopN (VnrRemap.cc:330-341; Eucclhyd:
+ * EucclhydRemap.cc:273-286), by cfg->schema. */
 ref_ctx* ref_create(const mhy_config* cfg, int* status);
 void ref_destroy(ref_ctx* r);
 
-/* One loop iteration (VnrRemap.cc:130-222) with the n/n+1 swap, no output.
+/* One loop iteration (VnrRemap.cc:130-222, EucclhydRemap.cc:84-168) with
+ * the n/n+1 swap, no output.
  * Returns MHY_OK or an error status. */
 int ref_step(ref_ctx* r);
 /* Whole reference loop with VTP/PVD output into output_dir (NULL/"none":
