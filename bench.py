@@ -274,7 +274,7 @@ def main():
     # graph is captured, and come before the warm-up
     tuning_steps = 0
     if cells >= (1 << 20):
-        while not ctx.tuned_kernels() and tuning_steps < 32:
+        while not ctx.tuned_kernels() and tuning_steps < 64:
             ctx.run(1)
             tuning_steps += 1
     progress("warm-up and timed steps")

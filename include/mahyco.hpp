@@ -557,10 +557,11 @@ class Vnr {
   double lastDump_ = 0.0;
 };
 
-/* eucclhyd_remap/Eucclhyd.h:40-371: the Eucclhyd driver's constructor, so
- * the reference's main compiles unchanged.  The library runs the VNR scheme
- * only: creating the context for this scheme fails with the library's
- * message and exit(1), as any fatal condition of the reference does. */
+/* eucclhyd_remap/Eucclhyd.h:40-371: the Eucclhyd driver.  The
+ * reference's signature builds the device context (mesh, Init.cc initial
+ * state, setUpTimeLoopN) for this scheme; simulate() runs the time loop
+ * (eucclhyd_remap/EucclhydRemap.cc:84-290) with its output (EucclhydRemap
+ * and the empty Particules files) into <output>/output. */
 class Eucclhyd {
  public:
   Eucclhyd(OptionsT* o, CstMeshT* cstmesh, GestTempsT* gt, TestT* test, CdlT* /*cdl*/, LimiteursT* lim,
@@ -569,6 +570,8 @@ class Eucclhyd {
       : varlp_(varlp), output_(output) {
     varlp_->create(optionsToConfig(MHY_SCHEMA_EUCCLHYD, o, cstmesh, gt, lim, eos, test));
   }
+  /* Over a context already created for the Eucclhyd scheme. */
+  Eucclhyd(VariablesLagRemap* varlp, const std::string& output) : varlp_(varlp), output_(output) {}
   void simulate() {
     mhy_ctx* x = varlp_->ctx();
     const std::string dir = (output_ == "none" || output_ == "NONE") ? std::string("none")

@@ -20,8 +20,9 @@ EXE = os.path.join(BIN_DIR, "mahyco")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ARCH = os.environ.get("MAHYCO_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["lagrange.hip", "sweep_wave.hip", "sweep_walk.hip", "sweep_cell.hip", "remap.hip", "context.hip"]
-HOST_SOURCES = ["host/donnees.cc", "host/json_input.cc", "host/init.cc", "host/output.cc"]
+HIP_SOURCES = ["lagrange.hip", "sweep_wave.hip", "sweep_walk.hip", "sweep_cell.hip", "remap.hip", "eucclhyd.hip",
+               "context.hip"]
+HOST_SOURCES = ["host/donnees.cc", "host/json_input.cc", "host/init.cc", "host/init_euc.cc", "host/output.cc"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]

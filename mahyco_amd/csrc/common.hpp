@@ -16,10 +16,26 @@
 namespace mhy {
 
 constexpr int NMAT = MHY_NBMATMAX;
-// live remap components: volume, mass, energy per material, then the
-// pseudo-viscosity content (reference vector index 3m+3; indices 3m..3m+2
-// carry momentum and kinetic energy, always zero in VNR cell vectors)
-constexpr int NCMAX = 3 * NMAT + 1;
+// Live components of a cell remap vector: volume, mass, energy per material
+// (3m), then the scheme's tail.  VNR: the pseudo-viscosity content
+// (reference vector index 3m+3; indices 3m..3m+2 carry momentum and kinetic
+// energy, always zero in VNR cell vectors).  Eucclhyd: the cell momentum
+// (3m, 3m+1), and the kinetic energy (3m+2) when the projection is
+// conservative; its pseudo-viscosity component is always zero.
+constexpr int NCMAX = 3 * NMAT + 3;
+
+// Layout of the remap vector, passed to the sweeps as their material
+// template parameter L: L = m (1..3) is the VNR vector; L = 4 + m the
+// Eucclhyd vector; L = 8 + m the Eucclhyd vector of a conservative
+// projection.
+template <int L>
+constexpr int kMat = L & 3;
+template <int L>
+constexpr bool kEuc = L > 3;
+template <int L>
+constexpr int kTail = L < 4 ? 1 : (L < 8 ? 2 : 3);
+template <int L>
+constexpr int kNC = 3 * kMat<L> + kTail<L>;
 
 // std::min / std::max argument conventions (cppnabla/types/MathFunctions.h:23-24)
 MHY_HD double smin(double a, double b) { return (b < a) ? b : a; }
@@ -108,8 +124,12 @@ MHY_HD double dkeySignedInv(unsigned long long k) {
 
 // Options and geometry constants, uniform for a launch.
 struct Opts {
-  int nx, ny, nbmat, nc;  // nc = 3*nbmat + 1 live remap components
+  int nx, ny, nbmat, nc;  // nc live remap components (kNC of the layout)
   int mode;
+  int euc;                // Eucclhyd scheme (cell-centred velocities)
+  int space_order;        // Eucclhyd: spaceOrder (1 or 2)
+  int conservative;       // Eucclhyd: projectionConservative
+  double deltat_min;
   int order;              // projectionOrder
   int limiter, limiter_pure, pp, pp_mixte, pp_complet;
   int pseudo_centree, methode, abort_energy;
@@ -169,6 +189,24 @@ struct KPT {
 
 using Fields = FieldsT<double>;
 using KP = KPT<double>;
+
+// Arrays of the Eucclhyd scheme (FP64), beside the shared fields: the cell
+// velocity, the velocity gradient, the sum over each cell's corners of the
+// material corner forces of the last step (the next step's material
+// pressure gradients, times the cell volume), the nodes' ranges of the
+// material pressures and of the cell velocity (the second-order
+// extrapolation), the cell centroid and volume ([2]: they move with the mesh
+// without remap) and perimeter, and the output velocities.
+struct EucFields {
+  double *vx[2], *vy[2];
+  double *gv[4];
+  double *fsx[NMAT], *fsy[NMAT];
+  double *pmin[NMAT], *pmax[NMAT];
+  double *vmin[2], *vmax[2];
+  double *xc[2], *yc[2], *ve[2];
+  double *perim;
+  double *outvx, *outvy;
+};
 
 // true for the mixed form (FP32 bulk storage)
 template <class R>

@@ -74,6 +74,10 @@ struct mhy_ctx {
   // members of type R) point to FP32 arrays and are retyped at launch
   Fields f;
   bool mixed = false;  // precision 1: FP32 bulk storage and sweep arithmetic
+  // the Eucclhyd scheme (eucclhyd.hip): its own Lagrange phase and final
+  // remap around the cell sweeps, no nodal remap
+  bool euc = false;
+  EucFields ef;
   HostInit init;
   hipStream_t stream = nullptr;
   int device = 0;
@@ -147,16 +151,19 @@ struct mhy_ctx {
   // walking bands; and, on steps whose first sweep runs along y, the
   // Lagrange cell update fused into that sweep or run as its own kernel.  On
   // a large problem the first steps time whole steps, one site after the
-  // other (two steps per candidate, so that both sweep orders are seen; the
-  // other sites keep their current choice), and keep the fastest; the
-  // choice is fixed before any graph is captured.
+  // other (the other sites keep their current choice): kTuneRounds rounds
+  // over the site's candidates, two steps per candidate and round (one of
+  // each sweep order), and keep the candidate with the fastest round, so
+  // that clock ramp-up in the first steps and single slow steps do not
+  // decide.  The choice is fixed before any graph is captured.
+  static constexpr int kTuneRounds = 2;
   struct Tuner {
     bool on = false;
     int step = 0, steps = 0;
     int ncand[3] = {0, 0, 0};  // site 0: nodal remap along y, 1: cell sweep along y, 2: y-first Lagrange
     int cand[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     int best[3] = {0, 0, 0};  // index into cand of the current choice
-    double ms[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double ms[3][kTuneRounds][3] = {};  // two-step time per site, round and candidate
     hipEvent_t ev[2] = {nullptr, nullptr};  // around the current step
     std::string choice;  // the kept variants, for reports
   } tune;
@@ -256,16 +263,19 @@ void tuneApply(mhy_ctx* x, int site, int ci) {
   else x->fuseLagY = (id == TL_FUSED);
 }
 
-// The site and candidate index of tuning step k (sites one after the other,
-// two steps per candidate); false past the schedule.
-bool tuneSlot(const mhy_ctx* x, int k, int& site, int& ci) {
+// The site, round and candidate index of tuning step k (sites one after the
+// other; within a site, rounds over the candidates, two steps each); false
+// past the schedule.
+bool tuneSlot(const mhy_ctx* x, int k, int& site, int& ci, int& round) {
   const auto& t = x->tune;
   for (site = 0; site < 3; ++site) {
-    if (k < 2 * t.ncand[site]) {
-      ci = k / 2;
+    const int per = 2 * t.ncand[site];
+    if (k < mhy_ctx::kTuneRounds * per) {
+      round = k / per;
+      ci = (k % per) / 2;
       return true;
     }
-    k -= 2 * t.ncand[site];
+    k -= mhy_ctx::kTuneRounds * per;
   }
   return false;
 }
@@ -275,30 +285,35 @@ bool tuneSlot(const mhy_ctx* x, int k, int& site, int& ci) {
 void tuneSelect(mhy_ctx* x) {
   auto& t = x->tune;
   if (!t.on) return;
-  int site, ci;
-  if (!tuneSlot(x, t.step, site, ci)) return;
+  int site, ci, round;
+  if (!tuneSlot(x, t.step, site, ci, round)) return;
   tuneApply(x, site, ci);
   (void)hipEventRecord(t.ev[0], x->stream);
 }
 
-// Ends a tuning step: adds its time to its candidate; after the last step
-// of a site, keeps the fastest candidate of that site.
+// Ends a tuning step: adds its time to its candidate's round; after the
+// last step of a site, keeps the candidate with the fastest round.
 void tuneStepDone(mhy_ctx* x) {
   auto& t = x->tune;
   if (!t.on) return;
-  int site, ci;
-  if (!tuneSlot(x, t.step, site, ci)) return;
+  int site, ci, round;
+  if (!tuneSlot(x, t.step, site, ci, round)) return;
   (void)hipEventRecord(t.ev[1], x->stream);
   float ms = 0.f;
   if (hipEventSynchronize(t.ev[1]) == hipSuccess && hipEventElapsedTime(&ms, t.ev[0], t.ev[1]) == hipSuccess)
-    t.ms[site][ci] += ms;
+    t.ms[site][round][ci] += ms;
   ++t.step;
-  int nsite, nci;
-  const bool more = tuneSlot(x, t.step, nsite, nci);
+  int nsite, nci, nround;
+  const bool more = tuneSlot(x, t.step, nsite, nci, nround);
   if (!more || nsite != site) {  // the site is done
+    auto fastest = [&](int c) {
+      double m = t.ms[site][0][c];
+      for (int r = 1; r < mhy_ctx::kTuneRounds; ++r) m = std::min(m, t.ms[site][r][c]);
+      return m;
+    };
     int b = 0;
     for (int c = 1; c < t.ncand[site]; ++c)
-      if (t.ms[site][c] < t.ms[site][b]) b = c;
+      if (fastest(c) < fastest(b)) b = c;
     t.best[site] = b;
     tuneApply(x, site, b);
   }
@@ -371,7 +386,80 @@ int checkLaunch(mhy_ctx* x, const char* what) {
   return MHY_OK;
 }
 
+// ------------------------------------------------ Eucclhyd phases
+using EucKernelFn = void (*)(KP, EucFields);
+using EucSweepFn = void (*)(KP);
+
+// State-of-step kernel and the time step (computeEOS .. computedeltat).
+int phaseEucStart(mhy_ctx* x) {
+  KP kp = makeKP(x);
+  static const EucKernelFn ks[3] = {k_euc_start<1>, k_euc_start<2>, k_euc_start<3>};
+  const unsigned nb = (unsigned)std::min<long>(kReduceBlocks, (x->ncell + kBlock - 1) / kBlock);
+  timed(x, K_DT, [&] {
+    hipLaunchKernelGGL(ks[x->o.nbmat - 1], dim3(nb), dim3(kBlock), 0, x->stream, kp, x->ef);
+    hipLaunchKernelGGL(k_euc_dt_finalize, dim3(1), dim3(64), 0, x->stream, kp);
+  });
+  return checkLaunch(x, "eucclhyd start");
+}
+
+// Node solver and cell update (extrapolateValue ..
+// updateCellCenteredLagrangeVariables).
+int phaseEucLagrange(mhy_ctx* x) {
+  KP kp = makeKP(x);
+  static const EucKernelFn kn[3] = {k_euc_node<1>, k_euc_node<2>, k_euc_node<3>};
+  static const EucKernelFn kc[3] = {k_euc_cell<1>, k_euc_cell<2>, k_euc_cell<3>};
+  const int mi = x->o.nbmat - 1;
+  timed(x, K_NODE, [&] { hipLaunchKernelGGL(kn[mi], gridFor(x->nnode), dim3(kBlock), 0, x->stream, kp, x->ef); });
+  timed(x, K_CELL, [&] { hipLaunchKernelGGL(kc[mi], gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp, x->ef); });
+  return checkLaunch(x, "eucclhyd lagrange");
+}
+
+// remapCellcenteredVariable
+int phaseEucFinal(mhy_ctx* x) {
+  KP kp = makeKP(x);
+  static const EucKernelFn kf[3] = {k_euc_final<1>, k_euc_final<2>, k_euc_final<3>};
+  timed(x, K_FINAL, [&] {
+    hipLaunchKernelGGL(kf[x->o.nbmat - 1], gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp, x->ef);
+  });
+  return checkLaunch(x, "eucclhyd final");
+}
+
+// The cell sweeps on the Eucclhyd vector layout (4 + m, or 8 + m with the
+// kinetic energy of a conservative projection): wave strips, or the LDS
+// tile kernel for the third order.
+template <int L>
+EucSweepFn eucSweepFor(bool xd, bool pp, bool o3) {
+  if (o3) return xd ? k_sweep_o3<double, L, true> : k_sweep_o3<double, L, false>;
+  if (pp) return xd ? k_sweep_wave<double, L, true, true> : k_sweep_wave<double, L, true, false>;
+  return xd ? k_sweep_wave<double, L, false, true> : k_sweep_wave<double, L, false, false>;
+}
+EucSweepFn eucSweepKernel(bool xd, int m, bool pp, bool conservative, bool o3) {
+  switch (m + (conservative ? 8 : 4)) {
+    case 5: return eucSweepFor<5>(xd, pp, o3);
+    case 6: return eucSweepFor<6>(xd, pp, o3);
+    case 7: return eucSweepFor<7>(xd, pp, o3);
+    case 9: return eucSweepFor<9>(xd, pp, o3);
+    case 10: return eucSweepFor<10>(xd, pp, o3);
+    default: return eucSweepFor<11>(xd, pp, o3);
+  }
+}
+
+int phaseEucSweep(mhy_ctx* x, int sweep, int stage) {
+  if (stage != MHY_STAGE_CELLS) return MHY_OK;  // no nodal remap
+  KP kp = makeKP(x);
+  kp.sweep = sweep;
+  const bool xd = (sweep == 1) ? x->x_then_y_n : !x->x_then_y_n;
+  kp.xdir = xd ? 1 : 0;
+  const bool o3 = (x->o.order == 3), pp = (x->o.pp == 1);
+  const dim3 grid = o3 ? sweepO3Grid(xd, x->o.nx, x->o.ny) : sweepWaveGrid(xd, x->o.nx, x->o.ny, false, pp);
+  const dim3 block(o3 ? kBlock : sweepWaveBlock(xd, false, pp));
+  const EucSweepFn k = eucSweepKernel(xd, x->o.nbmat, pp, x->o.conservative == 1, o3);
+  timed(x, xd ? K_SWEEP_X : K_SWEEP_Y, [&] { hipLaunchKernelGGL(k, grid, block, 0, x->stream, kp); });
+  return checkLaunch(x, "eucclhyd sweep");
+}
+
 int phaseDt(mhy_ctx* x) {
+  if (x->euc) return phaseEucStart(x);
   tuneSelect(x);  // a step starts here
   KP kp = makeKP(x);
   if (x->o.mode == MODE_ADVECTION) {
@@ -393,6 +481,7 @@ using KernelFnF = void (*)(KPT<float>);
 bool lagFused(const mhy_ctx* x) { return x->fuseLag && (x->x_then_y_n || x->fuseLagY); }
 
 int phaseLagrange(mhy_ctx* x) {
+  if (x->euc) return phaseEucLagrange(x);
   KP kp = makeKP(x);
   // (a grid's y dimension holds at most 65535 node rows)
   if (x->o.mode == MODE_FULL && x->o.nx >= 2 && x->o.ny >= 2 && x->o.ny - 1 <= 65535 && x->splitNode) {
@@ -454,6 +543,7 @@ KernelFn sweepKernel(bool wave, bool xd, int m, bool pp) {
 // One remap sweep: stage 1 = cell fluxes and update (computeGradPhiFace ..
 // computeUremap), stage 2 = nodal momentum remap (computeDualUremap).
 int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
+  if (x->euc) return phaseEucSweep(x, sweep, stage);
   KP kp = makeKP(x);
   kp.sweep = sweep;
   if (stage == MHY_STAGE_CELLS) {
@@ -541,6 +631,7 @@ int phaseSweep(mhy_ctx* x, int sweep) {
 }
 
 int phaseFinal(mhy_ctx* x) {
+  if (x->euc) return phaseEucFinal(x);
   if (x->finalDone) {  // already done by the cell stage of sweep 2
     x->finalDone = false;
     return MHY_OK;
@@ -621,6 +712,10 @@ double* fieldPtr(mhy_ctx* x, int field, long& n) {
     case MHY_CELL_SOUND_SPEED: return x->f.c[c];
     case MHY_CELL_DIVU: return x->f.divu[c];
     case MHY_CELL_MASS: return x->f.M;
+    case MHY_CELL_VELOCITY_X: return x->euc ? x->ef.vx[c] : nullptr;
+    case MHY_CELL_VELOCITY_Y: return x->euc ? x->ef.vy[c] : nullptr;
+    case MHY_CELL_OUT_VX: return x->euc ? x->ef.outvx : nullptr;
+    case MHY_CELL_OUT_VY: return x->euc ? x->ef.outvy : nullptr;
     default: break;
   }
   if (field >= MHY_MAT_FRACVOL && field < MHY_MAT_FRACVOL + MHY_NBMATMAX * MHY_MAT_STRIDE) {
@@ -642,11 +737,11 @@ double* fieldPtr(mhy_ctx* x, int field, long& n) {
   switch (field) {
     case MHY_NODE_VELOCITY_X: return x->f.ux[c];
     case MHY_NODE_VELOCITY_Y: return x->f.uy[c];
-    case MHY_NODE_MASS: return x->f.Mp;
+    case MHY_NODE_MASS: return x->euc ? nullptr : x->f.Mp;
     case MHY_NODE_COORD_X: return x->o.mode == MODE_LAGRANGE ? x->f.X[c] : nullptr;
     case MHY_NODE_COORD_Y: return x->o.mode == MODE_LAGRANGE ? x->f.Y[c] : nullptr;
-    case MHY_NODE_OUT_VX: return x->o.mode == MODE_LAGRANGE ? x->f.outvx : x->f.ux[c];
-    case MHY_NODE_OUT_VY: return x->o.mode == MODE_LAGRANGE ? x->f.outvy : x->f.uy[c];
+    case MHY_NODE_OUT_VX: return x->euc ? nullptr : (x->o.mode == MODE_LAGRANGE ? x->f.outvx : x->f.ux[c]);
+    case MHY_NODE_OUT_VY: return x->euc ? nullptr : (x->o.mode == MODE_LAGRANGE ? x->f.outvy : x->f.uy[c]);
     default: return nullptr;
   }
 }
@@ -658,6 +753,148 @@ double* fieldPtr(mhy_ctx* x, int field, long& n) {
 extern "C" {
 
 const char* mhy_last_error(const mhy_ctx* x) { return x ? x->err.c_str() : "null context"; }
+
+// Set-up of an Eucclhyd problem (Eucclhyd::simulate prologue,
+// eucclhyd_remap/EucclhydRemap.cc:273-286): options, initial state on the
+// host, device arrays (FP64), upload.  Returns MHY_OK or the failure code
+// (message in x->err).
+static int setupEucclhyd(mhy_ctx* x, const mhy_config& c, bool batchMember) {
+  if (batchMember) return fail(x, MHY_ERR_ARG, "the Eucclhyd scheme does not run in ensembles");
+  if (c.precision != 0) return fail(x, MHY_ERR_ARG, "the Eucclhyd scheme runs in FP64 only (precision 0)");
+  if (c.projection_order < 1 || c.projection_order > 3)
+    return fail(x, MHY_ERR_ARG, "projection order must be 1, 2 or 3");
+  if (c.projection_order == 3 && c.pp == 1)
+    return fail(x, MHY_ERR_ARG, "projection order 3 is not supported with the PenteBorne option");
+  if (c.sans_lagrange) return fail(x, MHY_ERR_ARG, "sansLagrange is a VNR option");
+  if (c.nbmat < 1 || c.nbmat > MHY_NBMATMAX) return fail(x, MHY_ERR_ARG, "1 to 3 materials");
+  if (c.space_order != 1 && c.space_order != 2) return fail(x, MHY_ERR_ARG, "spaceOrder must be 1 or 2");
+  for (int k = 0; k < c.nbmat; ++k)
+    if (c.eos[k] != MHY_EOS_PERFECT_GAS && c.eos[k] != MHY_EOS_VOID)
+      return fail(x, MHY_ERR_ARG,
+                  "equation of state " + std::to_string(c.eos[k]) +
+                      " is not programmed for the Eucclhyd scheme (PhaseLagrange.cc:111-135)");
+  if (c.nx < 1 || c.ny < 1 || (long)(c.nx + 1) * (c.ny + 1) >= (1L << 31))
+    return fail(x, MHY_ERR_ARG, "mesh size out of range (at most 2^31 nodes)");
+  x->euc = true;
+  x->mixed = false;
+  x->sweepWave = true;
+  x->fuseFinal = false, x->fuseLag = false, x->walkY = false;
+  x->tune.on = false;
+  std::string err;
+  EucInit u;
+  int st = buildEucInitialState(c, x->init, u, err);
+  if (st != MHY_OK) return fail(x, st, err);
+  if (c.avec_projection && !outerNormalsAxisExact(c, err)) return fail(x, MHY_ERR_ARG, err);
+  const HostInit& h = x->init;
+  x->ncell = (long)c.nx * c.ny;
+  x->nnode = (long)(c.nx + 1) * (c.ny + 1);
+  const long C = x->ncell, N = x->nnode;
+  const int m = c.nbmat;
+  Opts& o = x->o;
+  std::memset(&o, 0, sizeof(o));
+  o.nx = c.nx, o.ny = c.ny, o.nbmat = m;
+  o.nc = 3 * m + 2 + (c.projection_conservative == 1 ? 1 : 0);
+  o.mode = c.avec_projection ? MODE_FULL : MODE_LAGRANGE;
+  o.euc = 1;
+  o.space_order = c.space_order;
+  o.conservative = (c.projection_conservative == 1) ? 1 : 0;
+  o.deltat_min = c.deltat_min;
+  o.order = c.projection_order;
+  o.limiter = c.limiter, o.limiter_pure = c.limiter_pure;
+  o.pp = c.pp, o.pp_mixte = c.pp_mixte, o.pp_complet = c.pp_complet;
+  o.methode = -1;  // no nodal remap
+  for (int k = 0; k < MHY_NBMATMAX; ++k) o.eos[k] = c.eos[k], o.gammap[k] = c.gammap[k];
+  o.gamma = c.gamma, o.threshold = c.threshold, o.dx = c.dx, o.dy = c.dy;
+  const Boundary& b = h.bc;
+  o.bc_left = b.left, o.bc_right = b.right, o.bc_bottom = b.bottom, o.bc_top = b.top;
+  for (int d = 0; d < 2; ++d)
+    o.bcv_left[d] = b.vleft[d], o.bcv_right[d] = b.vright[d], o.bcv_bottom[d] = b.vbottom[d], o.bcv_top[d] = b.vtop[d];
+  o.flux_left = b.flux_left, o.flux_right = b.flux_right, o.flux_bottom = b.flux_bottom, o.flux_top = b.flux_top;
+  o.flux_any = (b.flux_left + b.flux_right + b.flux_bottom + b.flux_top) > 0;
+
+  hipError_t e = hipSuccess;
+  if ((e = hipGetDevice(&x->device)) != hipSuccess) return fail(x, MHY_ERR_DEVICE, hipGetErrorString(e));
+  if (!(x->scratch = scratchAddress(x->device)))
+    return fail(x, MHY_ERR_DEVICE, "hipGetSymbolAddress of the scratch array failed");
+  if ((e = hipStreamCreate(&x->stream)) != hipSuccess) return fail(x, MHY_ERR_DEVICE, hipGetErrorString(e));
+  Fields& f = x->f;
+  EucFields& ef = x->ef;
+  std::memset(&f, 0, sizeof(f));
+  std::memset(&ef, 0, sizeof(ef));
+#define AD(ptr, n) ((ptr) = allocD(x, (n), e))
+  for (int s = 0; s < 2; ++s) {
+    AD(f.rho[s], C), AD(f.e[s], C), AD(f.ux[s], N), AD(f.uy[s], N), AD(ef.vx[s], C), AD(ef.vy[s], C);
+    for (int k = 0; k < m; ++k) AD(f.rho_k[s][k], C), AD(f.e_k[s][k], C);
+  }
+  // pressure and sound speed: one array each (the reference's m_pressure,
+  // m_speed_velocity), formed at the start of every step
+  AD(f.p[0], C), AD(f.c[0], C);
+  f.p[1] = f.p[0], f.c[1] = f.c[0];
+  for (int k = 0; k < m; ++k) {
+    AD(f.p_k[0][k], C), AD(f.c_k[0][k], C);
+    f.p_k[1][k] = f.p_k[0][k], f.c_k[1][k] = f.c_k[0][k];
+    AD(f.f_k[k], C), AD(f.mu_k[k], C), AD(f.M_k[k], C);
+    AD(ef.fsx[k], C), AD(ef.fsy[k], C), AD(ef.pmin[k], N), AD(ef.pmax[k], N);
+  }
+  AD(f.M, C);
+  f.mixte = allocI(x, C, e);
+  f.pure = allocI(x, C, e);
+  for (int q = 0; q < 4; ++q) AD(ef.gv[q], C);
+  for (int d = 0; d < 2; ++d) AD(ef.vmin[d], N), AD(ef.vmax[d], N);
+  AD(ef.perim, C), AD(ef.outvx, C), AD(ef.outvy, C);
+  if (o.mode == MODE_LAGRANGE) {
+    for (int s = 0; s < 2; ++s) AD(ef.xc[s], C), AD(ef.yc[s], C), AD(ef.ve[s], C), AD(f.X[s], N), AD(f.Y[s], N);
+  } else {
+    AD(ef.xc[0], C), AD(ef.yc[0], C), AD(ef.ve[0], C);  // Euler cells: constant
+    ef.xc[1] = ef.xc[0], ef.yc[1] = ef.yc[0], ef.ve[1] = ef.ve[0];
+    for (int q = 0; q < o.nc; ++q) AD(f.UL[q], C), AD(f.U1[q], C), AD(f.U2[q], C);
+    AD(f.VL, C), AD(f.XcL, C), AD(f.YcL, C);
+  }
+  if (e == hipSuccess) {
+    void* p = nullptr;
+    if ((e = hipMalloc(&p, sizeof(Scalars))) == hipSuccess) x->allocs.push_back(p);
+    f.sc = static_cast<Scalars*>(p);
+  }
+#undef AD
+  if (e != hipSuccess) return fail(x, MHY_ERR_DEVICE, std::string("device allocation: ") + hipGetErrorString(e));
+  // initial state into the "n" slot 0 (setUpTimeLoopN, Init.cc:721-760)
+  int ok = MHY_OK;
+  auto up = [&](double* d, const std::vector<double>& v) {
+    if (ok == MHY_OK) ok = upload(x, d, v);
+  };
+  up(f.rho[0], h.rho), up(f.e[0], h.e), up(ef.vx[0], u.vx), up(ef.vy[0], u.vy);
+  for (int k = 0; k < m; ++k) {
+    up(f.rho_k[0][k], h.rho_k[k]), up(f.e_k[0][k], h.e_k[k]);
+    up(f.f_k[k], h.f_k[k]), up(f.mu_k[k], h.mu_k[k]);
+  }
+  up(ef.xc[0], u.xc), up(ef.yc[0], u.yc), up(ef.ve[0], u.ve), up(ef.perim, u.perim);
+  up(ef.outvx, u.outvx), up(ef.outvy, u.outvy);
+  if (o.mode == MODE_LAGRANGE) {
+    std::vector<double> X(N), Y(N);
+    for (long j = 0; j <= c.ny; ++j)
+      for (long i = 0; i <= c.nx; ++i) {
+        const long p = j * (c.nx + 1) + i;
+        X[p] = c.dx * (double)i, Y[p] = c.dy * (double)j;
+      }
+    up(f.X[0], X), up(f.Y[0], Y);
+  }
+  if (ok != MHY_OK) return ok;
+  if (hipMemcpy(f.mixte, h.mixte.data(), sizeof(int) * C, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(f.pure, h.pure.data(), sizeof(int) * C, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(x, MHY_ERR_DEVICE, "upload flags");
+  Scalars s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.t[0] = 0.0;
+  s0.dt[0] = h.deltat_init;
+  for (int q = 0; q < 2; ++q) s0.minKey[q] = dkeyPos(DBL_MAX), s0.maxKey[q] = dkeySigned(-DBL_MAX);
+  if (hipMemcpy(f.sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(x, MHY_ERR_DEVICE, "upload scalars");
+  x->init = HostInit();
+  x->cur = 0;
+  x->x_then_y_n = true;
+  x->iteration = 0;
+  return MHY_OK;
+}
 
 // batchMember: a problem of an ensemble, whose steps the batch launches
 // (with the stand-alone Lagrange cell kernel)
@@ -675,7 +912,13 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
     *out = x;
     return code;
   };
-  if (c.schema != MHY_SCHEMA_VNR) return bail(fail(x, MHY_ERR_ARG, "only the VNR scheme is supported"));
+  if (c.schema == MHY_SCHEMA_EUCCLHYD) {
+    const int st = setupEucclhyd(x, c, batchMember);
+    *out = x;
+    return st;
+  }
+  if (c.schema != MHY_SCHEMA_VNR)
+    return bail(fail(x, MHY_ERR_ARG, "the scheme must be VNR (2001) or Eucclhyd (2000)"));
   if (c.projection_order < 1 || c.projection_order > 3)
     return bail(fail(x, MHY_ERR_ARG, "projection order must be 1, 2 or 3"));
   if (c.projection_order == 3 && c.pp == 1)
@@ -739,7 +982,7 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
       t.cand[2][0] = TL_FUSED, t.cand[2][1] = TL_SEPARATE;
       t.ncand[2] = 2;
     }
-    t.steps = 2 * (t.ncand[0] + t.ncand[1] + t.ncand[2]);
+    t.steps = 2 * mhy_ctx::kTuneRounds * (t.ncand[0] + t.ncand[1] + t.ncand[2]);
     t.on = t.steps > 0;
     if (t.on && (hipEventCreate(&t.ev[0]) != hipSuccess || hipEventCreate(&t.ev[1]) != hipSuccess))
       return bail(fail(x, MHY_ERR_DEVICE, "tuner events"));
@@ -1061,6 +1304,7 @@ int mhy_get_field(mhy_ctx* x, int field, double* out, long long n) {
     return MHY_OK;
   }
   if (field == MHY_NODE_MASS) {
+    if (x->euc) return fail(x, MHY_ERR_ARG, "the Eucclhyd scheme has no node mass");
     KP kp = makeKP(x);
     launchR(x, k_node_mass<double>, k_node_mass<float>, gridFor(x->nnode), dim3(kBlock), kp);
     HIPCHK(x, hipStreamSynchronize(x->stream));
@@ -1133,6 +1377,7 @@ extern "C" {
 
 int mhy_conservation(mhy_ctx* x, double* sums) {
   if (!x || !sums) return MHY_ERR_ARG;
+  if (x->euc) return fail(x, MHY_ERR_ARG, "conservation sums are formed for the VNR scheme");
   const int m = x->o.nbmat, nq = 2 + m;
   const unsigned nb = consBlocks(x);
   double* d = nullptr;
@@ -1155,6 +1400,7 @@ int mhy_conservation(mhy_ctx* x, double* sums) {
 // several host threads or streams) race.
 int mhy_conservation_noalloc(mhy_ctx* x, double* sums) {
   if (!x || !sums) return MHY_ERR_ARG;
+  if (x->euc) return fail(x, MHY_ERR_ARG, "conservation sums are formed for the VNR scheme");
   const int m = x->o.nbmat, nq = 2 + m;
   const unsigned nb = consBlocks(x);
   const size_t need = (size_t)nb * nq * 4;
@@ -1192,6 +1438,34 @@ int mhy_kernel_bytes(const mhy_ctx* x, int k, double* bytes) {
   // p, c, mu_k, rho_k, e_k, p_k, c_k (bulk) and M, f_k, M_k, flags
   const double finalWrites = C * r * (5 + 5 * m) + C * d * (1 + 2 * m) + C * i4 * 2;
   double w = 0;
+  if (x->euc) {  // Eucclhyd kernels (FP64)
+    const bool o2 = x->o.space_order == 2, lag = x->o.mode == MODE_LAGRANGE;
+    // what the corner extrapolation reads of a cell: p_k, f_k, rho_k, the
+    // force sums, velocity, gradient, centroid, p, rho, c, volume
+    const double cellX = C * d * (5 * m + 12);
+    const double ranges = o2 ? N * d * (2 * m + 4) : 0.0;
+    switch (k) {
+      case K_DT:  // rho, e_k, rho_k, f_k, mu_k, c, volume, perimeter, velocity (+ node u) ;
+                  // p, c, p_k, c_k, gradient, M, M_k
+        w = C * d * (3 + 3 * m + 2 + 2) + (o2 ? N * 2 * d : 0.0) + C * d * (2 + 2 * m + (o2 ? 4 : 0) + 1 + m);
+        break;
+      case K_NODE: w = cellX + N * d * 2 + ranges + (lag ? N * d * 4 : 0.0); break;
+      case K_CELL:  // + M, M_k, mu_k, e_k and the node velocities and ranges ; U_L or state n+1, force sums
+        w = cellX + C * d * (1 + 3 * m) + N * d * 2 + ranges + C * d * 2 * m +
+            (lag ? C * d * (2 + 2 + 2 * m + 2 + 3 + 1) : C * d * (nc + 3));
+        break;
+      case K_SWEEP_X:
+      case K_SWEEP_Y:  // U_in, V_L and centroid, node velocities (+ flags) ; U_out
+        w = C * d * nc + C * d * 2 + N * 2 * d + C * i4 * 2 * (pp && x->o.pp_mixte) + C * d * nc;
+        break;
+      case K_FINAL:  // U_2, volume ; rho, e, velocity, f_k, mu_k, rho_k, e_k, outputs, flags
+        w = C * d * (nc + 1) + C * d * (2 + 2 + 4 * m + 2) + C * i4 * 2;
+        break;
+      default: w = 0.0; break;
+    }
+    *bytes = w;
+    return MHY_OK;
+  }
   switch (k) {
     case K_DT: w = C * (d + 2 * r) + N * 2 * r; break;  // vol ; c, p ; u
     case K_NODE: w = N * 2 * r + C * (2 * r + d) + N * 3 * r; break;  // u_n ; p, q ; M ; u_n+1, Mp
@@ -1287,6 +1561,15 @@ int mhy_simulate(mhy_ctx* x, const char* output_dir, int verbose) {
   auto dump = [&](double tn) -> int {
     nbCalls++;
     if (x->writer->disabled() || !(tn >= lastDump + x->cfg.output_time || tn == 0.)) return MHY_OK;
+    if (x->euc) {
+      // the Eucclhyd dump prints the pressure of state n, which the step
+      // forms at its start (dumpVariables after computeEOS,
+      // EucclhydRemap.cc:99-106); the EOS kernel is idempotent
+      KP kp = makeKP(x);
+      static const KernelFn ke[3] = {k_euc_eos<1>, k_euc_eos<2>, k_euc_eos<3>};
+      hipLaunchKernelGGL(ke[x->o.nbmat - 1], gridFor(x->ncell), dim3(kBlock), 0, x->stream, kp);
+      HIPCHK(x, hipStreamSynchronize(x->stream));
+    }
     int st = x->writer->write(x, nbCalls, tn);
     lastDump = tn;
     return st;
@@ -1314,7 +1597,7 @@ int mhy_simulate(mhy_ctx* x, const char* output_dir, int verbose) {
       std::printf("[%5d] t = %.8e DT = %.8e\n", x->iteration, tn, s.dt[1 - x->cur]);
     if (cont) endIteration(x);
   } while (cont);
-  {
+  if (!x->euc) {  // Vnr::simulate dumps once more; Eucclhyd::simulate does not
     Scalars s;
     int st = readScalars(x, s);
     if (st != MHY_OK) return st;
@@ -1335,6 +1618,7 @@ int ctxField(mhy_ctx* x, int field, std::vector<double>& out) {
 }
 int ctxNbmat(const mhy_ctx* x) { return x->o.nbmat; }
 int ctxProjection(const mhy_ctx* x) { return x->cfg.avec_projection; }
+int ctxEuc(const mhy_ctx* x) { return x->euc ? 1 : 0; }
 void ctxMesh(const mhy_ctx* x, long& nx, long& ny) { nx = x->o.nx, ny = x->o.ny; }
 }  // namespace mhy
 

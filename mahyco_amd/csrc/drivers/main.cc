@@ -1,6 +1,8 @@
 // The mahyco executable: same command line and output as the reference's
-// main (main.cc:13-80) for the VNR scheme, `mahyco donnees.txt` writing
-// ./output/VnrRemap.*.vtp and ./output/VnrRemap.pvd, on one MI355X.
+// main (main.cc:13-80), `mahyco donnees.txt` writing ./output/VnrRemap.*.vtp
+// and ./output/VnrRemap.pvd for the VNR scheme, ./output/EucclhydRemap.* and
+// the empty ./output/Particules.* files for the Eucclhyd scheme, on one
+// MI355X.
 // An optional second argument names the output base directory ("none"
 // disables output).
 //
@@ -87,8 +89,14 @@ int main(int argc, char* argv[]) {
     mahyco::LectureDonneesClass lecture;
     lecture.LectureDonnees(input, &cfg);
   }
+  if (cfg.schema == MHY_SCHEMA_EUCCLHYD) {
+    mahyco::VariablesLagRemap varlp(cfg);
+    mahyco::Eucclhyd c(&varlp, output);
+    c.simulate();
+    return 0;
+  }
   if (cfg.schema != MHY_SCHEMA_VNR) {
-    std::cerr << "[ERREUR] seul le schema VNR est disponible" << std::endl;
+    std::cerr << "[ERREUR] schema Lagrange inconnu" << std::endl;
     return 1;
   }
   mahyco::VariablesLagRemap varlp(cfg);

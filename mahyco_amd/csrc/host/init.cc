@@ -51,8 +51,17 @@ bool outerNormalsAxisExact(const mhy_config& o, std::string& err) {
     for (long i = 0; i < nx; ++i) {
       const P2 bl = node(j, i), br = node(j, i + 1), tr = node(j + 1, i + 1), tl = node(j + 1, i);
       P2 xc = {0.0, 0.0};
-      xc = add(xc, bl), xc = add(xc, br), xc = add(xc, tr), xc = add(xc, tl);
-      xc = mul(xc, 0.25);
+      if (o.schema == MHY_SCHEMA_EUCCLHYD) {  // polygon centroid (eucclhyd_remap/Init.cc:106-145)
+        const P2 X[4] = {bl, br, tr, tl};
+        double r = 0.0;
+        for (int q = 0; q < 4; ++q) r = r + cross(X[q], X[(q + 1) & 3]);
+        const double vol = 0.5 * r;
+        for (int q = 0; q < 4; ++q) xc = add(xc, mul(add(X[q], X[(q + 1) & 3]), cross(X[q], X[(q + 1) & 3])));
+        xc = mul(xc, 1.0 / (6.0 * vol));
+      } else {
+        xc = add(xc, bl), xc = add(xc, br), xc = add(xc, tr), xc = add(xc, tl);
+        xc = mul(xc, 0.25);
+      }
       // bottom, left, right, top face centres and the axis component of
       // their outer normals
       const P2 fc[4] = {mul(add(bl, br), 0.5), mul(add(bl, tl), 0.5), mul(add(br, tr), 0.5), mul(add(tl, tr), 0.5)};

@@ -34,8 +34,22 @@ struct HostInit {
 // message in err.
 int buildInitialState(const mhy_config& o, HostInit& s, std::string& err);
 
+// What an Eucclhyd problem adds: the cell velocity (and the output copy the
+// reference writes), the polygon centroid, volume and perimeter of the
+// set-up cells.
+struct EucInit {
+  std::vector<double> vx, vy, outvx, outvy, xc, yc, ve, perim;
+};
+
+// Initial state of an Eucclhyd problem (eucclhyd_remap/Init.cc): fills the
+// shared fields of s (densities, energies, fractions, flags, boundary
+// conditions, initial time step) and u.
+int buildEucInitialState(const mhy_config& o, HostInit& s, EucInit& u, std::string& err);
+
 // Outer face normals of the Euler cells (Init.cc:534-545) must be exactly
-// +-1 along their axis for the kernels' flux signs; checks every cell.
+// +-1 along their axis for the kernels' flux signs; checks every cell, with
+// the cell centre of the scheme (VNR: mean of the nodes; Eucclhyd: polygon
+// centroid).
 bool outerNormalsAxisExact(const mhy_config& o, std::string& err);
 
 }  // namespace mhy

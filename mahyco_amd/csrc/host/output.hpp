@@ -13,6 +13,7 @@ namespace mhy {
 int ctxField(mhy_ctx* x, int field, std::vector<double>& out);
 int ctxNbmat(const mhy_ctx* x);
 int ctxProjection(const mhy_ctx* x);
+int ctxEuc(const mhy_ctx* x);  // the Eucclhyd scheme
 void ctxMesh(const mhy_ctx* x, long& nx, long& ny);
 
 class OutputWriter {
@@ -25,7 +26,7 @@ class OutputWriter {
  private:
   std::string dir_;
   bool disabled_ = false;
-  std::map<double, std::string> files_;
+  std::map<double, std::string> files_, particles_;
 };
 
 }  // namespace mhy

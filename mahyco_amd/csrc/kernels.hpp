@@ -7,6 +7,19 @@
 
 namespace mhy {
 
+// Eucclhyd Lagrange phase and final remap (eucclhyd.hip); M: materials
+template <int M>
+__global__ void k_euc_start(KP kp, EucFields e);
+__global__ void k_euc_dt_finalize(KP kp);
+template <int M>
+__global__ void k_euc_eos(KP kp);
+template <int M>
+__global__ void k_euc_node(KP kp, EucFields e);
+template <int M>
+__global__ void k_euc_cell(KP kp, EucFields e);
+template <int M>
+__global__ void k_euc_final(KP kp, EucFields e);
+
 template <class R>
 __global__ void k_dt_reduce(KPT<R> kp);
 template <class R>

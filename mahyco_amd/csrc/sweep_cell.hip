@@ -360,10 +360,10 @@ constexpr int kO3YOut = kO3YRows - 6;
 
 // the faces and the update of the inner cell at along index a, across
 // index t, whose tile slot is tid (sY: NC planes of kO3Block means)
-template <class R, int M, bool XD>
+template <class R, int L, bool XD>
 __device__ __forceinline__ void sweepO3Cell(const KPT<R>& kp, const double* sY, const double* sH, int tid, long a,
                                             long t, long na) {
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
   constexpr int S = XD ? 1 : kO3YCols;  // thread distance of the along-sweep neighbour
   const Opts& o = kp.o;
   const auto& f = kp.f;
@@ -413,7 +413,7 @@ __device__ __forceinline__ void sweepO3Cell(const KPT<R>& kp, const double* sY, 
     phiM[q] = hasM ? o3Flux(swM, y, e[0], e[1], e[2]) : 0.0;
     phiP[q] = hasP ? o3Flux(swP, y + 1, e[1], e[2], e[3]) : 0.0;
   }
-  sweepUpdateCell<R, M, false, XD, double>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP);
+  sweepUpdateCell<R, L, false, XD, double>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP);
 }
 
 // One tile; blk: its linear block index.  R: bulk storage type.  The face
@@ -421,9 +421,10 @@ __device__ __forceinline__ void sweepO3Cell(const KPT<R>& kp, const double* sY, 
 // FP64; the mixed form stores the node velocities the widths come from in
 // FP32).  Every thread of the block reaches both barriers (no early
 // return), so the ensemble kernels can run several tiles in one workgroup.
-template <class R, int M, bool XD>
+// L: the remap-vector layout (kNC; VNR or Eucclhyd).
+template <class R, int L, bool XD>
 __device__ __forceinline__ void sweepO3Body(const KPT<R>& kp, unsigned blk) {
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
   __shared__ double sY[NC][kO3Block];
   __shared__ double sH[kO3Block];
   const Opts& o = kp.o;
@@ -449,7 +450,9 @@ __device__ __forceinline__ void sweepO3Body(const KPT<R>& kp, unsigned blk) {
   }
   const bool inT = (t < nt);
   double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
-  const double dt1 = f.sc->dt[kp.nxt];
+  // the time step of the end-of-Lagrange node positions (VNR: dt^{n+1};
+  // Eucclhyd: dt^n, eucclhyd_remap/PhaseLagrange.cc:573-583)
+  const double dt1 = f.sc->dt[kEuc<L> ? kp.cur : kp.nxt];
   if (inT) {
     const long b = min(max(a, 0L), na - 1);
     const long j = XD ? t : b, i = XD ? b : t;
@@ -460,7 +463,7 @@ __device__ __forceinline__ void sweepO3Body(const KPT<R>& kp, unsigned blk) {
     sH[tid] = widthCell<XD>(kp, dt1, j, i);
   }
   __syncthreads();
-  if (inT && a >= 0 && a < na && slot >= 3 && slot <= last - 3) sweepO3Cell<R, M, XD>(kp, sY[0], sH, tid, a, t, na);
+  if (inT && a >= 0 && a < na && slot >= 3 && slot <= last - 3) sweepO3Cell<R, L, XD>(kp, sY[0], sH, tid, a, t, na);
   __syncthreads();  // the ensemble kernels reuse the tile for their next block
 }
 
@@ -494,6 +497,18 @@ MHY_SWEEP_O3_INST(float, 1)
 MHY_SWEEP_O3_INST(float, 2)
 MHY_SWEEP_O3_INST(float, 3)
 #undef MHY_SWEEP_O3_INST
+
+// the Eucclhyd layouts (FP64 only, no ensembles)
+#define MHY_SWEEP_O3_EUC(L)                                  \
+  template __global__ void k_sweep_o3<double, L, true>(KP); \
+  template __global__ void k_sweep_o3<double, L, false>(KP);
+MHY_SWEEP_O3_EUC(5)
+MHY_SWEEP_O3_EUC(6)
+MHY_SWEEP_O3_EUC(7)
+MHY_SWEEP_O3_EUC(9)
+MHY_SWEEP_O3_EUC(10)
+MHY_SWEEP_O3_EUC(11)
+#undef MHY_SWEEP_O3_EUC
 
 #define MHY_SWEEP_CELL_INST(M, PP)                       \
   template __global__ void k_sweep_cell<M, PP, true>(KP); \

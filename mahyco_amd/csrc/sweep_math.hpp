@@ -157,24 +157,34 @@ __device__ __forceinline__ T sweptIntegral(const Profile<T>& pr, T h0, T ymoins,
 }
 
 // Phi of a cell at the start of the sweep: U/V, or the PenteBorne form
-// (fractions, densities, specific energies, Lagrange-phase pseudo), formed in
-// FP64 from the FP64 remap vector and then held as T; an FP32 U/V (mixed
-// form, limited slopes) is formed in FP32.
-template <int M, bool PP, class T>
+// (fractions, densities, specific energies, then the Lagrange-phase pseudo
+// of VNR or the velocity and specific kinetic energy of Eucclhyd: the
+// momentum components over the cell mass, PhaseRemap1.cc:530-547,
+// eucclhyd_remap/PhaseLagrange.cc:932-958), formed in FP64 from the FP64
+// remap vector and then held as T; an FP32 U/V (mixed form, limited slopes)
+// is formed in FP32.  L: the vector layout (kMat, kNC).
+template <int L, bool PP, class T>
 __device__ __forceinline__ void cellPhi(const double* u, double V, double qL, T* phi) {
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
+  constexpr int M = kMat<L>;
   if (PP) {
-    double sv = 0.0;
+    double sv = 0.0, sm = 0.0;
 #pragma unroll
     for (int k = 0; k < M; ++k) sv += u[k];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       phi[k] = T(u[k] / sv);
       phi[M + k] = T((u[k] != 0.) ? u[M + k] / u[k] : 0.0);
+      sm += u[M + k];
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) phi[2 * M + k] = T((u[M + k] != 0.) ? u[2 * M + k] / u[M + k] : 0.0);
-    phi[3 * M] = T(qL);
+    if constexpr (kEuc<L>) {
+#pragma unroll
+      for (int q = 3 * M; q < NC; ++q) phi[q] = T(u[q] / sm);
+    } else {
+      phi[3 * M] = T(qL);
+    }
   } else if (sizeof(T) == 4) {
     // FP32 slopes: Phi = U * (1/V) in FP32 (one reciprocal per cell)
     const T rV = T(1.0) / T(V);
@@ -187,11 +197,11 @@ __device__ __forceinline__ void cellPhi(const double* u, double V, double qL, T*
 }
 
 // mixed / pure indicators from a Phi vector (PhaseRemap1.cc:552-566)
-template <int M, class T>
+template <int L, class T>
 __device__ __forceinline__ int packedFlags(const T* phi) {
   int matcell = 0, imatpure = -1;
 #pragma unroll
-  for (int k = 0; k < M; ++k)
+  for (int k = 0; k < kMat<L>; ++k)
     if (phi[k] > T(0.)) matcell++, imatpure = k;
   const int mixte = (matcell > 1) ? 1 : 0;
   const int pure = (matcell > 1) ? -1 : imatpure;
@@ -228,18 +238,26 @@ __device__ __forceinline__ T faceDiffP(const Nbr<T>& n, int q) {
   return XD ? (n.phip[q] - n.phi[q]) / (n.xcp - n.xc) : (n.phi[q] - n.phip[q]) / (n.xc - n.xcp);
 }
 
-// Number of leading components whose PenteBorne swept integral is used.
-template <int M>
+// Number of leading components whose PenteBorne swept integral is used:
+// with the complete option every component (the Eucclhyd momentum and
+// kinetic energy are rescaled from their own integrals), else the volumes,
+// and the masses too in a pure neighbourhood.
+template <int L>
 __device__ __forceinline__ int ppIntegrated(const Opts& o, bool vp) {
-  return (o.pp_complet == 1) ? 3 * M : (vp ? 2 * M : M);
+  constexpr int M = kMat<L>;
+  return (o.pp_complet == 1) ? (kEuc<L> ? kNC<L> : 3 * M) : (vp ? 2 * M : M);
 }
 
-// Finishing of a PenteBorne flux vector (UtilesRemap-Impl.h:259-313, :436-487)
-template <int M, class T>
+// Finishing of a PenteBorne flux vector (UtilesRemap-Impl.h:259-313,
+// :436-487): masses and energies from the volume fluxes, the VNR
+// pseudo-viscosity with the volume flux, the Eucclhyd momentum and kinetic
+// energy with the mass flux.
+template <int L, class T>
 __device__ __forceinline__ void finishPP(const Opts& o, bool pureVariant, const T* phi, T* F) {
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
+  constexpr int M = kMat<L>;
   if (o.pp_complet == 1) {
-    T sfv = T(0.);
+    T sfv = T(0.), sfm = T(0.);
 #pragma unroll
     for (int k = 0; k < M; ++k) sfv += F[k];
     if (tabs(sfv) > T(o.threshold)) {
@@ -247,21 +265,33 @@ __device__ __forceinline__ void finishPP(const Opts& o, bool pureVariant, const 
       for (int k = 0; k < M; ++k) {
         if (!pureVariant) F[M + k] = (F[M + k] / sfv) * F[k];
         F[2 * M + k] = (F[2 * M + k] / sfv) * F[M + k];
+        sfm += F[M + k];
       }
-      F[3 * M] = phi[3 * M] * sfv;
+      if constexpr (kEuc<L>) {
+#pragma unroll
+        for (int q = 3 * M; q < NC; ++q) F[q] = (F[q] / sfv) * sfm;
+      } else {
+        F[3 * M] = phi[3 * M] * sfv;
+      }
     } else {
 #pragma unroll
       for (int q = 0; q < NC; ++q) F[q] = T(0.0);
     }
   } else {
-    T sfv = T(0.);
+    T sfv = T(0.), sfm = T(0.);
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       if (!pureVariant) F[M + k] = phi[M + k] * F[k];
       F[2 * M + k] = phi[2 * M + k] * F[M + k];
+      sfm += F[M + k];
       sfv += F[k];
     }
-    F[3 * M] = phi[3 * M] * sfv;
+    if constexpr (kEuc<L>) {
+#pragma unroll
+      for (int q = 3 * M; q < NC; ++q) F[q] = phi[q] * sfm;
+    } else {
+      F[3 * M] = phi[3 * M] * sfv;
+    }
   }
 }
 
@@ -281,10 +311,11 @@ __device__ __forceinline__ T gradQ(const Opts& o, const Nbr<T>& n, int q) {
 // plus face (Av, type 1), and the dual flux (PB) of the mass components.
 // vminus / vplus: normal velocities of the minus / plus faces.  WANT_AR /
 // WANT_AV select the face fluxes the caller uses (the other stays zero).
-template <int M, bool XD, class T, bool WANT_AR = true, bool WANT_AV = true>
+template <int L, bool XD, class T, bool WANT_AR = true, bool WANT_AV = true>
 __device__ __forceinline__ void ppFluxes(const Opts& o, const Nbr<T>& n, T vminus, T vplus, T dtn, bool dual, T* Ar,
                                          T* Av, T* Dual) {
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
+  constexpr int M = kMat<L>;
   const int lz = n.vp ? o.limiter_pure : o.limiter;
   const T h0 = n.h0, hp = n.hp, hm = n.hm;
   const T thr = T(o.threshold);
@@ -298,9 +329,9 @@ __device__ __forceinline__ void ppFluxes(const Opts& o, const Nbr<T>& n, T vminu
   for (int q = 0; q < NC; ++q) Ar[q] = Av[q] = T(0.0);
 #pragma unroll
   for (int k = 0; k < M; ++k) Dual[k] = T(0.0);
-  const int nInt = ppIntegrated<M>(o, n.vp);
+  const int nInt = ppIntegrated<L>(o, n.vp);
 #pragma unroll
-  for (int q = 0; q < 3 * M; ++q) {
+  for (int q = 0; q < (kEuc<L> ? NC : 3 * M); ++q) {
     const bool needF = (WANT_AR || WANT_AV) && q < nInt;
     const bool needD = dual && q >= M && q < 2 * M;
     if (!needF && !needD) continue;
@@ -315,13 +346,13 @@ __device__ __forceinline__ void ppFluxes(const Opts& o, const Nbr<T>& n, T vminu
     if (needD) Dual[q - M] = flat ? phi * ppdv : sweptIntegral<T>(pr, h0, phim, phip, T(0.), ppdv);
   }
   if (WANT_AR) {
-    finishPP<M, T>(o, n.vp, n.phi, Ar);
+    finishPP<L, T>(o, n.vp, n.phi, Ar);
     if (ppvR == T(0.))
 #pragma unroll
       for (int q = 0; q < NC; ++q) Ar[q] = T(0.0);
   }
   if (WANT_AV) {
-    finishPP<M, T>(o, n.vp, n.phi, Av);
+    finishPP<L, T>(o, n.vp, n.phi, Av);
     if (ppvV == T(0.))
 #pragma unroll
       for (int q = 0; q < NC; ++q) Av[q] = T(0.0);

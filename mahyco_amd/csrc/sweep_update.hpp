@@ -28,12 +28,13 @@ __device__ __forceinline__ void twoSumAcc(double& s, double& comp, double v) {
 // PB).  The flux of the last face of a line is stored here.  Uown: the
 // cell's remap vector at the start of the sweep when the caller holds it in
 // registers (the first sweep with the Lagrange update fused), else null and
-// it is read from U_L / U_1.
-template <class R, int M, bool PP, bool XD, class T>
+// it is read from U_L / U_1.  L: the vector layout (kMat, kNC).
+template <class R, int L, bool PP, bool XD, class T>
 __device__ __forceinline__ void sweepUnew(const KPT<R>& kp, long c, long i, long j, long a, bool hasM, bool hasP,
                                           double vm, double vp, double Lm, double Lp, const T* phiM, const T* phiP,
                                           double* Unew, double* Gdual, const double* Uown = nullptr) {
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
+  constexpr int M = kMat<L>;
   const Opts& o = kp.o;
   const auto& f = kp.f;
   const long nx = o.nx, ny = o.ny;
@@ -106,19 +107,25 @@ __device__ __forceinline__ void sweepUnew(const KPT<R>& kp, long c, long i, long
 }
 
 // The update of cell c and its stores: the final remap of the cell from
-// registers (second sweep, fused) or the new remap vector, and the flux of
-// its minus face for the dual remap.
-template <class R, int M, bool PP, bool XD, class T>
+// registers (second sweep, fused; VNR) or the new remap vector, and the flux
+// of its minus face for the dual remap.
+template <class R, int L, bool PP, bool XD, class T>
 __device__ __forceinline__ void sweepUpdateCell(const KPT<R>& kp, long c, long i, long j, long a, bool hasM,
                                                 bool hasP, double vm, double vp, double Lm, double Lp,
                                                 const T* phiM, const T* phiP, const double* Uown = nullptr) {
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
+  constexpr int M = kMat<L>;
   const auto& f = kp.f;
   double Unew[NC], Gd[M];
-  sweepUnew<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, Gd, Uown);
-  if (kp.fuseFinal) {
-    finalCell<R, M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
-  } else {
+  sweepUnew<R, L, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, Gd, Uown);
+  bool stored = false;
+  if constexpr (!kEuc<L>) {
+    if (kp.fuseFinal) {
+      finalCell<R, M>(kp, c, Unew);  // remapVariables on the cell, U2 stays in registers
+      stored = true;
+    }
+  }
+  if (!stored) {
     double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
 #pragma unroll
     for (int q = 0; q < NC; ++q) Uout[q][c] = Unew[q];
@@ -136,34 +143,35 @@ __device__ __forceinline__ void sweepUpdateCell(const KPT<R>& kp, long c, long i
 // used: a run-time choice between its members makes the compiler merge the
 // two paths' accesses through a selected address, which puts the struct in
 // scratch memory.
-template <class R, int M>
+template <class R, int L>
 struct CellOut {
-  FinalOut<R, M> fin;
-  double U[3 * M + 1];
-  double G[M];
-  double DF[M];
+  FinalOut<R, kMat<L>> fin;
+  double U[kNC<L>];
+  double G[kMat<L>];
+  double DF[kMat<L>];
 };
 
-template <class R, int M, bool FUSE>
-__device__ __forceinline__ void storeCellOut(const KPT<R>& kp, long c, long fm, const CellOut<R, M>& out, bool withG) {
+template <class R, int L, bool FUSE>
+__device__ __forceinline__ void storeCellOut(const KPT<R>& kp, long c, long fm, const CellOut<R, L>& out, bool withG) {
   const auto& f = kp.f;
   if constexpr (FUSE) {
-    finalStore<R, M>(kp, c, out.fin);
+    finalStore<R, kMat<L>>(kp, c, out.fin);
   } else {
     double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
 #pragma unroll
-    for (int q = 0; q < 3 * M + 1; ++q) Uout[q][c] = out.U[q];
+    for (int q = 0; q < kNC<L>; ++q) Uout[q][c] = out.U[q];
   }
   if (withG) {
 #pragma unroll
-    for (int k = 0; k < M; ++k) f.G[k][fm] = out.G[k];
+    for (int k = 0; k < kMat<L>; ++k) f.G[k][fm] = out.G[k];
   }
 }
 
 // Visits the members of out that a launch stores, in a fixed order: the
 // final fields (FUSE) or the remap vector, then G (withG), then DF (withDF).
-template <class R, int M, bool FUSE, class Fn>
-__device__ __forceinline__ void visitCellOut(CellOut<R, M>& out, bool withG, bool withDF, Fn&& fn) {
+template <class R, int L, bool FUSE, class Fn>
+__device__ __forceinline__ void visitCellOut(CellOut<R, L>& out, bool withG, bool withDF, Fn&& fn) {
+  constexpr int M = kMat<L>;
   if constexpr (FUSE) {
     FinalOut<R, M>& fo = out.fin;
     fn(fo.rho), fn(fo.e), fn(fo.q), fn(fo.p), fn(fo.c), fn(fo.mixte), fn(fo.pure), fn(fo.Mc);
@@ -171,7 +179,7 @@ __device__ __forceinline__ void visitCellOut(CellOut<R, M>& out, bool withG, boo
     for (int k = 0; k < M; ++k) fn(fo.fr[k]), fn(fo.Mk[k]), fn(fo.mu[k]), fn(fo.rk[k]), fn(fo.ek[k]), fn(fo.pk[k]), fn(fo.ck[k]);
   } else {
 #pragma unroll
-    for (int q = 0; q < 3 * M + 1; ++q) fn(out.U[q]);
+    for (int q = 0; q < kNC<L>; ++q) fn(out.U[q]);
   }
   if (withG) {
 #pragma unroll
@@ -188,14 +196,19 @@ template <class R, int M>
 constexpr int kFuse8 = 1 + 2 * M + (sizeof(R) == 8 ? 5 + 5 * M : 0);
 template <class R, int M>
 constexpr int kFuse4 = 2 + (sizeof(R) == 4 ? 5 + 5 * M : 0);
-// LDS doubles per staged cell: the larger of the two layouts, each followed
-// by G and DF
-template <class R, int M, bool PP>
-constexpr int kStageBytes = ((kFuse8<R, M> + M + (PP ? M : 0)) * 8 + kFuse4<R, M> * 4) > (3 * M + 1 + M + (PP ? M : 0)) * 8
-                                ? (kFuse8<R, M> + M + (PP ? M : 0)) * 8 + kFuse4<R, M> * 4
-                                : (3 * M + 1 + M + (PP ? M : 0)) * 8;
-template <class R, int M, bool PP>
-constexpr int kStageWords = (kStageBytes<R, M, PP> + 7) / 8;
+// LDS bytes per staged cell: the larger of the two layouts (final fields or
+// remap vector), each followed by G and DF
+template <class R, int L, bool PP>
+constexpr int kStageFuseBytes = (kFuse8<R, kMat<L>> + kMat<L> + (PP ? kMat<L> : 0)) * 8 + kFuse4<R, kMat<L>> * 4;
+template <class R, int L, bool PP>
+constexpr int kStageVecBytes = (kNC<L> + kMat<L> + (PP ? kMat<L> : 0)) * 8;
+// (the Eucclhyd layouts never stage the final fields)
+template <class R, int L, bool PP>
+constexpr int kStageBytes = (!kEuc<L> && kStageFuseBytes<R, L, PP> > kStageVecBytes<R, L, PP>)
+                                ? kStageFuseBytes<R, L, PP>
+                                : kStageVecBytes<R, L, PP>;
+template <class R, int L, bool PP>
+constexpr int kStageWords = (kStageBytes<R, L, PP> + 7) / 8;
 
 // LDS staging of the CellOut of a tile of TILE cells, in a buffer of
 // kStageWords * TILE doubles: 8-byte member w of element e at
@@ -204,24 +217,25 @@ template <int TILE>
 struct OutStage {
   double* s8;
   unsigned* s4;
-  template <class R, int M, bool FUSE>
+  template <class R, int L, bool FUSE>
   __device__ __forceinline__ static OutStage make(double* buf, bool withG, bool withDF) {
-    const int n8 = (FUSE ? kFuse8<R, M> : 3 * M + 1) + (withG ? M : 0) + (withDF ? M : 0);
+    constexpr int M = kMat<L>;
+    const int n8 = (FUSE ? kFuse8<R, M> : kNC<L>) + (withG ? M : 0) + (withDF ? M : 0);
     return OutStage{buf, reinterpret_cast<unsigned*>(buf + n8 * TILE)};
   }
-  template <class R, int M, bool FUSE>
-  __device__ __forceinline__ void put(CellOut<R, M>& out, int e, bool withG, bool withDF) const {
+  template <class R, int L, bool FUSE>
+  __device__ __forceinline__ void put(CellOut<R, L>& out, int e, bool withG, bool withDF) const {
     int w8 = 0, w4 = 0;
-    visitCellOut<R, M, FUSE>(out, withG, withDF, [&](auto& v) {
+    visitCellOut<R, L, FUSE>(out, withG, withDF, [&](auto& v) {
       using V = typename std::remove_reference<decltype(v)>::type;
       if constexpr (sizeof(V) == 8) s8[(w8++) * TILE + e] = __builtin_bit_cast(double, v);
       else s4[(w4++) * TILE + e] = __builtin_bit_cast(unsigned, v);
     });
   }
-  template <class R, int M, bool FUSE>
-  __device__ __forceinline__ void get(CellOut<R, M>& out, int e, bool withG, bool withDF) const {
+  template <class R, int L, bool FUSE>
+  __device__ __forceinline__ void get(CellOut<R, L>& out, int e, bool withG, bool withDF) const {
     int w8 = 0, w4 = 0;
-    visitCellOut<R, M, FUSE>(out, withG, withDF, [&](auto& v) {
+    visitCellOut<R, L, FUSE>(out, withG, withDF, [&](auto& v) {
       using V = typename std::remove_reference<decltype(v)>::type;
       if constexpr (sizeof(V) == 8) v = __builtin_bit_cast(V, s8[(w8++) * TILE + e]);
       else v = __builtin_bit_cast(V, s4[(w4++) * TILE + e]);

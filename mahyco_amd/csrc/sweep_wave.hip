@@ -87,13 +87,14 @@ __device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, kWav
 // one cell, consecutive threads on consecutive cells of a row: every store
 // instruction writes whole 128-byte lines of each field.  Every thread of the
 // block calls this (two barriers).  FUSE: the second sweep (kp.fuseFinal).
-template <class R, int M, bool PP, class T, bool FUSE>
+template <class R, int L, bool PP, class T, bool FUSE>
 __device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, long j, long a, long bx, long by,
                                              int slot, int wave, int lane, bool output, bool hasM, bool hasP,
                                              double vm, double vp, double Lm, double Lp, const T* phiM,
                                              const T* phiP, const double* dfv, bool pb, double* stage,
                                              const double* Uown) {
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
+  constexpr int M = kMat<L>;
   const Opts& o = kp.o;
   const auto& f = kp.f;
   const long nx = o.nx, ny = o.ny;
@@ -101,11 +102,11 @@ __device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, l
   constexpr int TW = W * kYCols;    // tile width (columns)
   constexpr int TILE = kYOut * TW;  // cells per tile, fewer than the block's threads
   const bool withG = (o.methode == 0 || o.methode == 1);
-  const OutStage<TILE> st = OutStage<TILE>::template make<R, M, FUSE>(stage, withG, pb);
+  const OutStage<TILE> st = OutStage<TILE>::template make<R, L, FUSE>(stage, withG, pb);
   if (output) {
-    CellOut<R, M> out;
+    CellOut<R, L> out;
     double Unew[NC];
-    sweepUnew<R, M, PP, false, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, out.G, Uown);
+    sweepUnew<R, L, PP, false, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, Unew, out.G, Uown);
     if constexpr (FUSE) {
       finalCompute<R, M>(kp, c, Unew, out.fin);
     } else {
@@ -114,17 +115,17 @@ __device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, l
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) out.DF[k] = dfv[k];
-    st.template put<R, M, FUSE>(out, (slot - kHalo) * TW + wave * kYCols + lane % kYCols, withG, pb);
+    st.template put<R, L, FUSE>(out, (slot - kHalo) * TW + wave * kYCols + lane % kYCols, withG, pb);
   }
   __syncthreads();
   const int e = threadIdx.x;
   if (e < TILE) {
     const long row = by * kYOut + e / TW, col = bx * TW + e % TW;
     if (row < ny && col < nx) {
-      CellOut<R, M> out;
-      st.template get<R, M, FUSE>(out, e, withG, pb);
+      CellOut<R, L> out;
+      st.template get<R, L, FUSE>(out, e, withG, pb);
       const long cc = row * nx + col;
-      storeCellOut<R, M, FUSE>(kp, cc, cc, out, withG);  // the minus face of a y-cell has the cell's index
+      storeCellOut<R, L, FUSE>(kp, cc, cc, out, withG);  // the minus face of a y-cell has the cell's index
       if (pb) {
 #pragma unroll
         for (int k = 0; k < M; ++k) f.DF[k][cc] = out.DF[k];
@@ -137,11 +138,13 @@ __device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, l
 // b: linear block index.  Along x the strips of all rows are numbered
 // row-major and each wave of the block takes the next one; along y a block
 // covers 16 columns x 12 rows, blocks ordered column-group-major per row band.
-template <class R, int M, bool PP, bool XD, bool LAG>
+// L: the remap-vector layout (kMat, kNC; VNR or Eucclhyd).
+template <class R, int L, bool PP, bool XD, bool LAG>
 __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   // type of the slope / limiter / PenteBorne arithmetic
   using T = typename std::conditional<kMixed<R> && !PP, float, double>::type;
-  constexpr int NC = 3 * M + 1;
+  constexpr int NC = kNC<L>;
+  constexpr int M = kMat<L>;
   constexpr int S = XD ? 1 : kYCols;  // lane distance of the along-sweep neighbour
   const Opts& o = kp.o;
   const auto& f = kp.f;
@@ -177,7 +180,10 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   const bool output = in && slot >= kHalo && slot <= lastSlot - kHalo;
 
   double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
-  const double dtn = f.sc->dt[kp.cur], dt1 = f.sc->dt[kp.nxt];
+  // dt1: the time step that moved the nodes to their end-of-Lagrange
+  // positions (VNR: dt^{n+1}, updatePosition; Eucclhyd: dt^n,
+  // eucclhyd_remap/PhaseLagrange.cc:573-583)
+  const double dtn = f.sc->dt[kp.cur], dt1 = f.sc->dt[kEuc<L> ? kp.cur : kp.nxt];
   const bool needH = PP || o.limiter >= MHY_LIM_MINMODG;
   const bool mixteOpt = PP && o.pp_mixte == 1;
   const bool pb = PP && o.methode == 2;
@@ -192,6 +198,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     double u[NC];
     double V, qv = 0.0;
     if constexpr (LAG) {
+      static_assert(!kEuc<L>, "the fused Lagrange sweep is the VNR cell update");
       // the Lagrange update of the cell, its remap vector in registers; only
       // an updated cell stores what outlives the step
       LagCell<M> lc;
@@ -204,13 +211,14 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
       if (output && PP) f.qL[cb] = lc.q;                // PenteBorne Phi of the second sweep
       if (output && kp.geomLoad) f.VL[cb] = lc.V, f.XcL[cb] = lc.xc.x, f.YcL[cb] = lc.xc.y;
     } else {
-      const int nload = PP ? 3 * M : NC;
+      // (the VNR PenteBorne Phi takes the pseudo from q_L, not from U)
+      const int nload = (PP && !kEuc<L>) ? 3 * M : NC;
 #pragma unroll
       for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cb] : 0.0;
       // the Lagrange volume and centroid: stored by the cell update, or
       // formed again from the end-of-Lagrange node positions (bitwise the
       // values the cell update formed)
-      if (kp.geomLoad) {
+      if (kEuc<L> || kp.geomLoad) {  // (Eucclhyd: always stored)
         V = f.VL[cb];
         x = XD ? f.XcL[cb] : f.YcL[cb];
       } else {
@@ -218,12 +226,12 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
         lagGeometry(kp, dt1, jb, ib, V, xcv);
         x = XD ? xcv.x : xcv.y;
       }
-      if (PP) qv = f.qL[cb];
+      if (PP && !kEuc<L>) qv = f.qL[cb];
     }
-    cellPhi<M, PP, T>(u, PP ? 1.0 : V, qv, ph);
+    cellPhi<L, PP, T>(u, PP ? 1.0 : V, qv, ph);
     h = needH ? T(widthW<XD>(kp, dt1, jb, ib)) : T(0.0);
     flg = 0;
-    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<M, T>(ph);
+    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<L, T>(ph);
   };
   T phi[NC], H = T(0.0);
   double xc = 0.0;
@@ -330,7 +338,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 #pragma unroll
     for (int k = 0; k < M; ++k) Dual[k] = T(0.0);
     const bool formed = o.order > 1;  // the fluxes are only formed for order > 1
-    if (formed && in) ppFluxes<M, XD, T>(o, n, T(vm), T(vp), T(dtn), pb && output, Ar, Av, Dual);
+    if (formed && in) ppFluxes<L, XD, T>(o, n, T(vm), T(vp), T(dtn), pb && output, Ar, Av, Dual);
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
       const T avm = shfl(Av[q], lm), arp = shfl(Ar[q], lp);
@@ -351,16 +359,21 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 #pragma unroll
       for (int k = 0; k < M; ++k) f.DF[k][c] = dfv[k];
     }
-    sweepUpdateCell<R, M, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, LAG ? uOwn : nullptr);
+    sweepUpdateCell<R, L, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, LAG ? uOwn : nullptr);
   } else {
     // one buffer for both sweep orders: kStageWords doubles per tile cell
-    __shared__ double stage[kStageWords<R, M, PP> * kYOut * kYWaves<R, PP> * kYCols];
+    __shared__ double stage[kStageWords<R, L, PP> * kYOut * kYWaves<R, PP> * kYCols];
     const double* Uown = LAG ? uOwn : nullptr;
-    if (kp.fuseFinal)
-      sweepYStaged<R, M, PP, T, true>(kp, c, i, j, a, bx, by, slot, wave, lane, output, hasM, hasP, vm, vp, Lm, Lp,
-                                      phiM, phiP, dfv, pb, stage, Uown);
-    else
-      sweepYStaged<R, M, PP, T, false>(kp, c, i, j, a, bx, by, slot, wave, lane, output, hasM, hasP, vm, vp, Lm, Lp,
+    bool fused = false;
+    if constexpr (!kEuc<L>) {  // the fused final remap is the VNR one
+      if (kp.fuseFinal) {
+        sweepYStaged<R, L, PP, T, true>(kp, c, i, j, a, bx, by, slot, wave, lane, output, hasM, hasP, vm, vp, Lm, Lp,
+                                        phiM, phiP, dfv, pb, stage, Uown);
+        fused = true;
+      }
+    }
+    if (!fused)
+      sweepYStaged<R, L, PP, T, false>(kp, c, i, j, a, bx, by, slot, wave, lane, output, hasM, hasP, vm, vp, Lm, Lp,
                                        phiM, phiP, dfv, pb, stage, Uown);
   }
 }
@@ -417,6 +430,21 @@ MHY_SWEEP_WAVE_INST_R(double)
 MHY_SWEEP_WAVE_INST_R(float)
 #undef MHY_SWEEP_WAVE_INST_R
 #undef MHY_SWEEP_WAVE_INST
+
+// the Eucclhyd layouts (FP64 only): momentum (4 + m), and kinetic energy
+// (8 + m) for a conservative projection
+#define MHY_SWEEP_WAVE_EUC(L)                                          \
+  template __global__ void k_sweep_wave<double, L, false, true>(KP);  \
+  template __global__ void k_sweep_wave<double, L, false, false>(KP); \
+  template __global__ void k_sweep_wave<double, L, true, true>(KP);   \
+  template __global__ void k_sweep_wave<double, L, true, false>(KP);
+MHY_SWEEP_WAVE_EUC(5)
+MHY_SWEEP_WAVE_EUC(6)
+MHY_SWEEP_WAVE_EUC(7)
+MHY_SWEEP_WAVE_EUC(9)
+MHY_SWEEP_WAVE_EUC(10)
+MHY_SWEEP_WAVE_EUC(11)
+#undef MHY_SWEEP_WAVE_EUC
 
 // threads per block of the wave sweeps: 4 waves along x; along y 4, or 8
 // in the mixed form's limited-slope sweeps

@@ -50,7 +50,7 @@ def test_tuned_run_equals_untuned(name, precision):
     try:
         assert tuned.tuned_kernels() == ""  # still tuning
         done = 0
-        for chunk in (3, 11, 6):  # tuning steps (at most 14), then graph replays
+        for chunk in (3, 25, 6):  # tuning steps (at most 28), then graph replays
             tuned.run(chunk)
             plain.run(chunk)
             done += chunk
