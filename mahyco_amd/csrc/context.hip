@@ -157,6 +157,7 @@ struct mhy_ctx {
   // that clock ramp-up in the first steps and single slow steps do not
   // decide.  The choice is fixed before any graph is captured.
   static constexpr int kTuneRounds = 2;
+  static constexpr double kTuneMargin = 0.01;  // relative gain a non-default candidate must show
   struct Tuner {
     bool on = false;
     int step = 0, steps = 0;
@@ -311,9 +312,12 @@ void tuneStepDone(mhy_ctx* x) {
       for (int r = 1; r < mhy_ctx::kTuneRounds; ++r) m = std::min(m, t.ms[site][r][c]);
       return m;
     };
+    // another candidate replaces the default (index 0) only for a clear
+    // gain: candidates within the step-to-step noise keep the default, so
+    // that repeated runs keep the same kernels
     int b = 0;
     for (int c = 1; c < t.ncand[site]; ++c)
-      if (fastest(c) < fastest(b)) b = c;
+      if (fastest(c) < (1.0 - mhy_ctx::kTuneMargin) * fastest(b)) b = c;
     t.best[site] = b;
     tuneApply(x, site, b);
   }

@@ -144,28 +144,38 @@ def test_fields_bitwise_equal_to_oracle(name):
         orc.close()
 
 
-def test_sedov_1024_pp_mixte_properties():
-    """Full-size run (1024^2 cells on the unit square, 300 steps) checked
-    through what the scheme preserves: total mass (the boundaries are walls or
-    imposed zero velocity), fractions in [0, 1], finite positive densities and
-    energies.  (A spacing of 1.2/1024 is refused at creation: there the
-    polygon centroids put some outer face normals off +-1, which the sweeps'
-    flux signs assume.)"""
-    cfg = _from("SEDOV_Mono_Mat_Superbee_PenteBorne_mixte", nx=1024, ny=1024, dx=1.0 / 1024, dy=1.0 / 1024)
-    log(f"[euc full size] sedov 1024^2")
+def test_bisod_1024_pp_mixte_properties():
+    """Full-size run (two-material Sod, 1024^2 cells on the unit square,
+    PenteBorne-mixte, 300 steps) checked through what the scheme preserves:
+    each material's mass (walls all round) to 1e-8 relative, fractions in
+    [0, 1] summing to 1, finite positive densities and energies.  The remap's
+    final stage rescales each cell's material volumes to the cell volume and
+    renormalises the fractions (eucclhyd_remap/PhaseRemapFinal.cc:17-232),
+    so material masses are kept only to rounding drift: the oracle drifts by
+    2e-9 relative on this case at 256^2 over 300 steps.  (The reference fixes the first
+    time step of its cases, 1e-4 here, about the CFL step of this mesh; a
+    Sedov case at this size starts far above it and stops on the NaN check,
+    in the reference as here.)"""
+    cfg = _from("SOD_X_bi_mat_Superbee_PenteBorne_mix", nx=1024, ny=1024, dx=1.0 / 1024, dy=1.0 / 1024)
+    log("[euc full size] bi-material Sod 1024^2")
     ctx = mahyco_amd.Context(cfg)
     try:
         vol = cfg.dx * cfg.dy
-        m0 = np.sum(ctx.get("density")) * vol
+
+        def masses():
+            return [np.sum(ctx.get("fracvol", k) * ctx.get("density_env", k)) * vol for k in range(cfg.nbmat)]
+
+        m0 = masses()
         ctx.run(300)
-        rho = ctx.get("density")
-        e = ctx.get("energy")
-        f = ctx.get("fracvol", 0)
+        m1 = masses()
+        rho, e = ctx.get("density"), ctx.get("energy")
         assert np.all(np.isfinite(rho)) and np.all(np.isfinite(e))
         assert np.all(rho > 0) and np.all(e > 0)
-        assert np.all((f >= 0) & (f <= 1))
-        m1 = np.sum(rho) * vol
-        assert abs(m1 - m0) <= 1e-12 * m0, (m0, m1)
+        fs = [ctx.get("fracvol", k) for k in range(cfg.nbmat)]
+        assert all(np.all((f >= 0) & (f <= 1)) for f in fs)
+        assert np.max(np.abs(sum(fs) - 1.0)) <= 1e-12
+        for a, b in zip(m0, m1):
+            assert abs(b - a) <= 1e-8 * a, (m0, m1)
         ts = ctx.time_state()
         assert ts.iteration == 300 and ts.t_n > 0
     finally:
