@@ -17,26 +17,26 @@ namespace sw {
 
 // std::min / std::max argument conventions
 template <class T>
-__device__ __forceinline__ T tmin(T a, T b) {
+__host__ __device__ __forceinline__ T tmin(T a, T b) {
   return (b < a) ? b : a;
 }
 template <class T>
-__device__ __forceinline__ T tmax(T a, T b) {
+__host__ __device__ __forceinline__ T tmax(T a, T b) {
   return (a < b) ? b : a;
 }
 template <class T>
-__device__ __forceinline__ T tabs(T a) {
+__host__ __device__ __forceinline__ T tabs(T a) {
   return fabs(a);
 }
 // includes/Freefunctions.h:86-92
 template <class T>
-__device__ __forceinline__ T tdivide(T a, T b) {
+__host__ __device__ __forceinline__ T tdivide(T a, T b) {
   return (tabs(b) < T(1.0E-12)) ? T(0.0) : a / b;
 }
 
 // remap/UtilesRemap.cc:9-23
 template <class T>
-__device__ __forceinline__ T fluxLimiter(int lim, T r) {
+__host__ __device__ __forceinline__ T fluxLimiter(int lim, T r) {
   if (lim == MHY_LIM_MINMOD) return tmax(T(0.0), tmin(T(1.0), r));
   if (lim == MHY_LIM_SUPERBEE) return tmax(T(0.0), tmax(tmin(T(2.0) * r, T(1.0)), tmin(r, T(2.0))));
   if (lim == MHY_LIM_VANLEER) return (r <= T(0.0)) ? T(0.0) : T(2.0) * r / (T(1.0) + r);
@@ -45,8 +45,8 @@ __device__ __forceinline__ T fluxLimiter(int lim, T r) {
 
 // remap/UtilesRemap.cc:25-72
 template <class T>
-__device__ __forceinline__ T fluxLimiterPP(int lim, T gradplus, T gradmoins, T yplus, T ymoins, T h0, T hplus,
-                                           T hmoins) {
+__host__ __device__ __forceinline__ T fluxLimiterPP(int lim, T gradplus, T gradmoins, T yplus, T ymoins, T h0,
+                                                    T hplus, T hmoins) {
   T grady = T(0.0);
   if (gradplus * gradmoins < T(0.0)) return T(0.);
   if (lim == MHY_LIM_MINMODG) {
@@ -73,9 +73,25 @@ __device__ __forceinline__ T fluxLimiterPP(int lim, T gradplus, T gradmoins, T y
   return grady;
 }
 
-// remap/UtilesRemap-Impl.h:99-126, one component
+// remap/UtilesRemap-Impl.h:99-126, one component:
+//   (limiter(tdivide(gp, gm)) * gm + limiter(tdivide(gm, gp)) * gp) / 2.
+// With minmod one quotient suffices.  The quotient whose numerator has the
+// larger magnitude is, correctly rounded, at least 1 in magnitude, where
+// minmod gives exactly 1 for a positive and exactly +0 for a negative
+// argument (and tdivide gives 0 when the other operand is below 1e-12), so
+// only the other quotient is formed.  Both limiter values, and so the
+// result, are bitwise those of the two-quotient form, NaN and infinite
+// operands included (the result is then NaN in both forms).
 template <class T>
-__device__ __forceinline__ T limitedGradient(int lim, T gp, T gm, T phip, T phim, T h0, T hp, T hm) {
+__host__ __device__ __forceinline__ T limitedGradient(int lim, T gp, T gm, T phip, T phim, T h0, T hp, T hm) {
+  if (lim == MHY_LIM_MINMOD) {
+    const bool pSmall = tabs(gp) < tabs(gm);
+    const T num = pSmall ? gp : gm, den = pSmall ? gm : gp;
+    const T q = fluxLimiter<T>(lim, tdivide(num, den));  // |num| <= |den| (or a NaN)
+    const T s = (tabs(num) < T(1.0E-12) || ((num > T(0.0)) != (den > T(0.0)))) ? T(0.0) : T(1.0);
+    const T lp = pSmall ? q : s, lm = pSmall ? s : q;  // limiter(gp/gm), limiter(gm/gp)
+    return (lp * gm + lm * gp) / T(2.0);
+  }
   if (lim < MHY_LIM_MINMODG)
     return (fluxLimiter<T>(lim, tdivide(gp, gm)) * gm + fluxLimiter<T>(lim, tdivide(gm, gp)) * gp) / T(2.0);
   return fluxLimiterPP<T>(lim, gp, gm, phip, phim, h0, hp, hm);
