@@ -238,6 +238,26 @@ __device__ __forceinline__ unsigned xcdBlock(unsigned b, unsigned nblocks) {
 }
 __device__ __forceinline__ unsigned xcdBlock() { return xcdBlock(blockIdx.x, gridDim.x); }
 
+// DPP lane moves (a VALU operand modifier, no LDS round trip): wave_shr:1
+// gives each lane the value of lane - 1, wave_shl:1 that of lane + 1; the
+// lane with no source (0, resp. 63) keeps its own value, as a shuffle clamped
+// to the lane itself does.  Called in converged control flow only.
+constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
+template <int CTRL>
+__device__ __forceinline__ int dppMove(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dppMove(float v) {
+  return __int_as_float(dppMove<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dppMove(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dppMove<CTRL>((int)(b & 0xffffffffLL)), hi = dppMove<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // One launch over an ensemble of problems: the launch parameters of problem
 // p for step variant v are table[v * nprob + p]; the virtual blocks of problem
 // p are start[p] .. start[p+1]-1, laid out as rows of gx[p] blocks, and

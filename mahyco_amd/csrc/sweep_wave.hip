@@ -80,6 +80,21 @@ __device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src
 __device__ __forceinline__ float shfl(float v, int src) { return __shfl(v, src, kWave); }
 __device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, kWave); }
 
+// value of the along-sweep minus / plus neighbour lane (lm / lp: its index,
+// or the lane itself at the wave's ends).  Along x the neighbours are the
+// adjacent lanes (DPP); along y they are kYCols lanes away (shuffle).
+// Called in converged control flow only.
+template <bool XD, class V>
+__device__ __forceinline__ V nbrM(V v, int lm) {
+  if constexpr (XD) return dppMove<kDppWaveShr1>(v);
+  else return shfl(v, lm);
+}
+template <bool XD, class V>
+__device__ __forceinline__ V nbrP(V v, int lp) {
+  if constexpr (XD) return dppMove<kDppWaveShl1>(v);
+  else return shfl(v, lp);
+}
+
 }  // namespace
 
 // Along y a lane's row segment is only 4 cells wide, so the block stages the
@@ -188,9 +203,12 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   const bool mixteOpt = PP && o.pp_mixte == 1;
   const bool pb = PP && o.methode == 2;
 
-  // the cell's remap vector at the sweep start, kept when the Lagrange
-  // update forms it in registers (no copy in memory)
-  double uOwn[NC];
+  // LAG: the cell's remap vector at the sweep start, formed by the Lagrange
+  // update (no copy in HBM).  It is parked in LDS, one slot per thread, from
+  // the update to the end of the sweep, so that it holds no registers through
+  // the slope computation; each thread reads back only its own slots.
+  constexpr int kThreads = XD ? 256 : 64 * kYWaves<R, PP>;
+  __shared__ double uPark[LAG ? NC : 1][LAG ? kThreads : 1];
   // one cell of this lane's line at along index b: Phi, centroid, width, flags
   auto loadCell = [&](long b, T* ph, double& x, T& h, int& flg) {
     const long jb = XD ? t : b, ib = XD ? b : t;
@@ -204,7 +222,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
       LagCell<M> lc;
       lagrangeCellCompute<R, M>(kp, cb, jb, ib, output, lc);
 #pragma unroll
-      for (int q = 0; q < NC; ++q) u[q] = uOwn[q] = lc.U[q];
+      for (int q = 0; q < NC; ++q) u[q] = uPark[q][threadIdx.x] = lc.U[q];
       V = lc.V, qv = lc.q;
       x = XD ? lc.xc.x : lc.xc.y;
       if (output && o.methode == 1) f.rL[cb] = lc.rho;  // A2 dual fluxes
@@ -248,15 +266,15 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   T phm[NC], php[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
-    const T vm = shfl(phi[q], lm), vp = shfl(phi[q], lp);
+    const T vm = nbrM<XD>(phi[q], lm), vp = nbrP<XD>(phi[q], lp);
     phm[q] = inM ? vm : phi[q];
     php[q] = inP ? vp : phi[q];
   }
-  double xcm = shfl(xc, lm), xcp = shfl(xc, lp);
+  double xcm = nbrM<XD>(xc, lm), xcp = nbrP<XD>(xc, lp);
   xcm = inM ? xcm : xc, xcp = inP ? xcp : xc;
-  T Hm = shfl(H, lm), Hp = shfl(H, lp);
+  T Hm = nbrM<XD>(H, lm), Hp = nbrP<XD>(H, lp);
   Hm = inM ? Hm : H, Hp = inP ? Hp : H;
-  int fm = shfl(fl, lm), fp = shfl(fl, lp);
+  int fm = nbrM<XD>(fl, lm), fp = nbrP<XD>(fl, lp);
   fm = inM ? fm : fl, fp = inP ? fp : fl;
   // FP64: absolute centroids (the reference's differences); FP32: offsets
   // from this cell's centroid, formed in FP64
@@ -274,7 +292,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     for (int q = 0; q < NC; ++q) dM[q] = faceDiffM<XD, T>(n, q);
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
-      const T v = shfl(dM[q], lp);
+      const T v = nbrP<XD>(dM[q], lp);
       dP[q] = inP ? v : T(0.0);
     }
   } else {
@@ -317,7 +335,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 #pragma unroll
     for (int q = 0; q < NC; ++q) g[q] = in ? gradQ<XD, T>(o, n, q) : T(0.0);
 #pragma unroll
-    for (int q = 0; q < NC; ++q) gm[q] = shfl(g[q], lm), gp[q] = shfl(g[q], lp);
+    for (int q = 0; q < NC; ++q) gm[q] = nbrM<XD>(g[q], lm), gp[q] = nbrP<XD>(g[q], lp);
     // face between along-cells lo (lower index) and hi
     auto faceValue = [&](long b, const T* pLo, const T* gLo, double xLo, const T* pHi, const T* gHi, double xHi,
                          double v, T* out) {
@@ -341,7 +359,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     if (formed && in) ppFluxes<L, XD, T>(o, n, T(vm), T(vp), T(dtn), pb && output, Ar, Av, Dual);
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
-      const T avm = shfl(Av[q], lm), arp = shfl(Ar[q], lp);
+      const T avm = nbrM<XD>(Av[q], lm), arp = nbrP<XD>(Ar[q], lp);
       if (formed) {
         phiM[q] = hasM ? avm - Ar[q] : T(0.0);
         phiP[q] = hasP ? Av[q] - arp : T(0.0);
@@ -352,6 +370,11 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 #pragma unroll
       for (int k = 0; k < M; ++k) dfv[k] = formed ? (double)Dual[k] * w : 0.0;
     }
+  }
+  double uOwn[NC];
+  if constexpr (LAG) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) uOwn[q] = uPark[q][threadIdx.x];
   }
   if constexpr (XD) {
     if (!output) return;
@@ -387,17 +410,24 @@ constexpr int kMinWaves = (!XD && kMixed<R> && !PP) ? 4 : 1;
 // the same for the ensemble kernels, whose three-material form would spill
 template <class R, int M, bool PP, bool XD>
 constexpr int kMinWavesBatch = (M < 3) ? kMinWaves<R, PP, XD> : 1;
+// the plain sweeps of one material in FP64 with limited slopes need 98
+// VGPRs unbounded; 96 (5 waves per SIMD instead of 4) costs no spill and runs
+// the 4096^2 y sweep 22 % faster.  (The fused Lagrange sweeps would spill.)
+template <class R, int L, bool PP, bool XD>
+constexpr int kMinWavesPlain = (!kMixed<R> && L == 1 && !PP) ? 5 : kMinWaves<R, PP, XD>;
+template <class R, int L, bool PP, bool XD>
+constexpr int kMinWavesLag = (!kMixed<R> && L == 1 && !PP) ? 5 : kMinWaves<R, PP, XD>;
 
 template <class R, int M, bool PP, bool XD>
 __global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>))
-__attribute__((amdgpu_waves_per_eu(kMinWaves<R, PP, XD>))) void k_sweep_wave(KPT<R> kp) {
+__attribute__((amdgpu_waves_per_eu(kMinWavesPlain<R, M, PP, XD>))) void k_sweep_wave(KPT<R> kp) {
   sweepWaveBody<R, M, PP, XD, false>(kp, XD ? blockIdx.x : xcdBlock());
 }
 
 // the first sweep of a step with the cell part of the Lagrange phase
 template <class R, int M, bool PP, bool XD>
 __global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>))
-__attribute__((amdgpu_waves_per_eu(kMinWaves<R, PP, XD>))) void k_sweep_wave_lag(KPT<R> kp) {
+__attribute__((amdgpu_waves_per_eu(kMinWavesLag<R, M, PP, XD>))) void k_sweep_wave_lag(KPT<R> kp) {
   sweepWaveBody<R, M, PP, XD, true>(kp, XD ? blockIdx.x : xcdBlock());
 }
 
