@@ -141,6 +141,11 @@ struct Opts {
   double bcv_left[2], bcv_right[2], bcv_bottom[2], bcv_top[2];
   int periodic_right;
   int flux_left, flux_right, flux_bottom, flux_top, flux_any;
+  // one material, perfect gas, Lagrange + remap: the material fields
+  // (fraction, mass fraction, mass, density, energy, pressure, sound speed)
+  // are stored only for cells whose envx flag is set; elsewhere they equal
+  // 1, 1 and the cell's mean fields bit for bit
+  int compact1;
 };
 
 // Device pointers of one problem (structure of arrays, cell index j*nx+i,
@@ -170,6 +175,7 @@ struct FieldsT {
   double *DF[NMAT];  // PB dual flux per cell of the current sweep
   double *UD1[3];    // nodal vector after dual sweep 1 (M ux, M uy, M)
   double *XL2;       // end-of-Lagrange node coordinate along the second sweep
+  unsigned char* envx;  // compact1: the cell's material fields are stored (1) or derived (0)
   Scalars* sc;
 };
 
@@ -207,6 +213,13 @@ struct EucFields {
   double *perim;
   double *outvx, *outvy;
 };
+
+// bitwise equality (distinguishes signed zeros, never true for a NaN pattern
+// compared with another)
+__device__ __forceinline__ bool sameBits(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
+__device__ __forceinline__ bool sameBits(float a, float b) { return __float_as_int(a) == __float_as_int(b); }
 
 // true for the mixed form (FP32 bulk storage)
 template <class R>

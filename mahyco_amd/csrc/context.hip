@@ -704,6 +704,30 @@ bool isBulkField(int field) {
   return true;
 }
 
+bool isMaterialField(int field) {
+  return field >= MHY_MAT_FRACVOL && field < MHY_MAT_FRACVOL + MHY_NBMATMAX * MHY_MAT_STRIDE;
+}
+
+// compact1: writes the derived material fields of the cells that do not
+// store them into the material arrays (the flags stay), so that the arrays
+// hold every cell's values
+int expandMaterialFields(mhy_ctx* x) {
+  if (!x->o.compact1) return MHY_OK;
+  KP kp = makeKP(x);
+  launchR(x, k_env_expand<double>, k_env_expand<float>, gridFor(x->ncell), dim3(kBlock), kp);
+  HIPCHK(x, hipGetLastError());
+  return MHY_OK;
+}
+
+// compact1: expands the material fields and marks every cell as storing them
+int storeMaterialFields(mhy_ctx* x) {
+  if (!x->o.compact1) return MHY_OK;
+  const int st = expandMaterialFields(x);
+  if (st != MHY_OK) return st;
+  HIPCHK(x, hipMemsetAsync(x->f.envx, 1, x->ncell, x->stream));
+  return MHY_OK;
+}
+
 // device pointer and length of a field id (state n of the reference)
 double* fieldPtr(mhy_ctx* x, int field, long& n) {
   const int c = x->cur;
@@ -1027,6 +1051,12 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
   o.periodic_right = b.periodic_right;
   o.flux_left = b.flux_left, o.flux_right = b.flux_right, o.flux_bottom = b.flux_bottom, o.flux_top = b.flux_top;
   o.flux_any = (b.flux_left + b.flux_right + b.flux_bottom + b.flux_top) > 0;
+  // one perfect-gas material, Lagrange + remap: the final remap stores the
+  // material fields only where they differ from the means (MAHYCO_COMPACT=0
+  // stores them everywhere, as for several materials)
+  o.compact1 = !batchMember && m == 1 && o.mode == MODE_FULL && o.eos[0] == MHY_EOS_PERFECT_GAS;
+  if (const char* cp = std::getenv("MAHYCO_COMPACT"))
+    if (std::string(cp) == "0") o.compact1 = 0;
 
   hipError_t e = hipSuccess;
   if ((e = hipGetDevice(&x->device)) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, hipGetErrorString(e)));
@@ -1055,6 +1085,11 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
   AD(f.M, C);
   f.mixte = allocI(x, C, e);
   f.pure = allocI(x, C, e);
+  if (o.compact1 && e == hipSuccess) {
+    void* p = nullptr;
+    if ((e = hipMalloc(&p, C)) == hipSuccess) x->allocs.push_back(p);
+    f.envx = static_cast<unsigned char*>(p);
+  }
   AR(f.Mp, N);
   if (o.periodic_right) {
     AR(f.rho0, C), AR(f.e0, C);
@@ -1120,6 +1155,8 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
   if (hipMemcpy(f.mixte, h.mixte.data(), sizeof(int) * C, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(f.pure, h.pure.data(), sizeof(int) * C, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(x, MHY_ERR_DEVICE, "upload flags"));
+  // the initial material fields are stored: every cell reads them
+  if (o.compact1 && hipMemset(f.envx, 1, C) != hipSuccess) return bail(fail(x, MHY_ERR_DEVICE, "material flags"));
   Scalars s0;
   std::memset(&s0, 0, sizeof(s0));
   s0.t[0] = 0.0;
@@ -1298,6 +1335,10 @@ int mhy_get_mesh_size(const mhy_ctx* x, int* nx, int* ny) {
 
 int mhy_get_field(mhy_ctx* x, int field, double* out, long long n) {
   if (!x || !out) return MHY_ERR_ARG;
+  if (isMaterialField(field)) {
+    const int st = expandMaterialFields(x);
+    if (st != MHY_OK) return st;
+  }
   HIPCHK(x, hipStreamSynchronize(x->stream));
   if (field == MHY_CELL_MIXTE || field == MHY_CELL_PURE) {
     if (n != x->ncell) return fail(x, MHY_ERR_ARG, "size");
@@ -1340,6 +1381,10 @@ int mhy_set_field(mhy_ctx* x, int field, const double* in, long long n) {
   double* p = fieldPtr(x, field, len);
   if (!p || field == MHY_NODE_MASS) return fail(x, MHY_ERR_ARG, "field not settable");
   if (n != len) return fail(x, MHY_ERR_ARG, "size");
+  // a compact context stores every cell's material fields from here on, so
+  // that setting a mean field leaves the material fields as they were
+  const int st = storeMaterialFields(x);
+  if (st != MHY_OK) return st;
   HIPCHK(x, hipStreamSynchronize(x->stream));
   if (x->mixed && isBulkField(field)) {
     std::vector<float> h(in, in + len);

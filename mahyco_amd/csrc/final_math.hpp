@@ -171,6 +171,18 @@ __device__ __forceinline__ void finalStore(const KPT<R>& kp, long c, const Final
   f.M[c] = out.Mc;
   f.mixte[c] = out.mixte;
   f.pure[c] = out.pure;
+  if constexpr (M == 1) {
+    if (kp.o.compact1) {
+      // the material fields equal 1, 1 and the mean fields bit for bit in
+      // every cell but exceptional ones (a negative density or energy, an
+      // empty cell): only those store them
+      const bool derived = sameBits(out.fr[0], 1.0) && sameBits(out.mu[0], R(1.0)) && sameBits(out.Mk[0], out.Mc) &&
+                           sameBits(out.rk[0], out.rho) && sameBits(out.ek[0], out.e) &&
+                           sameBits(out.pk[0], out.p) && sameBits(out.ck[0], out.c);
+      f.envx[c] = derived ? 0 : 1;
+      if (derived) return;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     f.f_k[k][c] = out.fr[k];

@@ -91,5 +91,7 @@ template <class R>
 __global__ void k_node_mass(KPT<R> kp);
 template <class R>
 __global__ void k_conservation(KPT<R> kp, double* out);
+template <class R>
+__global__ void k_env_expand(KPT<R> kp);
 
 }  // namespace mhy

@@ -105,18 +105,28 @@ __device__ __forceinline__ bool lagrangeCellCompute(const KPT<R>& kp, long c, lo
   // the cell's step-start words, loaded up front: their uses below are
   // conditional, and unconditional loads can all be in flight together
   double fk[NMAT], Mk0[NMAT], rk0[NMAT], qk0[NMAT], ek0[NMAT], pk0[NMAT], mu0[NMAT];
-#pragma unroll
-  for (int k = 0; k < m; ++k) {
-    fk[k] = f.f_k[k][c];
-    Mk0[k] = f.M_k[k][c];
-    rk0[k] = (double)f.rho_k[cur][k][c];
-    qk0[k] = (double)f.q_k[cur][k][c];
-    ek0[k] = (double)f.e_k[cur][k][c];
-    pk0[k] = (double)f.p_k[cur][k][c];
-    mu0[k] = (double)f.mu_k[k][c];
-  }
   const double rhoS = (double)f.rho[cur][c], divuS = (double)f.divu[cur][c], cS = (double)f.c[cur][c];
   const double vol0 = f.vol[cur][c], Mc0 = f.M[c];
+  // one material (compact1): fraction and mass fraction 1, the material
+  // mass, density, energy and pressure those of the cell, unless the final
+  // remap stored them (envx)
+  const bool derived = (m == 1) && o.compact1 && !f.envx[c];
+  if (derived) {
+    fk[0] = 1.0, mu0[0] = 1.0, Mk0[0] = Mc0, rk0[0] = rhoS;
+    ek0[0] = (double)f.e[cur][c], pk0[0] = (double)f.p[cur][c];
+    qk0[0] = (double)f.q_k[cur][0][c];
+  } else {
+#pragma unroll
+    for (int k = 0; k < m; ++k) {
+      fk[k] = f.f_k[k][c];
+      Mk0[k] = f.M_k[k][c];
+      rk0[k] = (double)f.rho_k[cur][k][c];
+      qk0[k] = (double)f.q_k[cur][k][c];
+      ek0[k] = (double)f.e_k[cur][k][c];
+      pk0[k] = (double)f.p_k[cur][k][c];
+      mu0[k] = (double)f.mu_k[k][c];
+    }
+  }
   D2 X[4];
   lagNodes(kp, dt1, j, i, X);
   const double V = lagVolume(X);

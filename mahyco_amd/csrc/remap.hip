@@ -210,6 +210,24 @@ constexpr int kDXOut = kDW - 6;
 constexpr int kDYOut = kDYRows - 6;
 __device__ __forceinline__ double dshfl(double v, int src) { return __shfl(v, src, kDW); }
 __device__ __forceinline__ float dshfl(float v, int src) { return __shfl(v, src, kDW); }
+// value of the node OFF (|OFF| <= 2) strip positions along the sweep.  Along
+// x, DPP moves by one lane (twice for two); along y, a shuffle of OFF *
+// kDYCols lanes.  A lane whose source lies outside the wave keeps its own
+// value, except that along x a lane one position from the wave's end gets the
+// end lane's value for OFF = -2 / +2: both are halo lanes of the strip, whose
+// gradients no updated node reads.
+template <bool XD, int OFF, class V>
+__device__ __forceinline__ V dnbr(V v, int lane) {
+  if constexpr (XD) {
+    if constexpr (OFF == -2) return dppMove<kDppWaveShr1>(dppMove<kDppWaveShr1>(v));
+    else if constexpr (OFF == -1) return dppMove<kDppWaveShr1>(v);
+    else if constexpr (OFF == 1) return dppMove<kDppWaveShl1>(v);
+    else return dppMove<kDppWaveShl1>(dppMove<kDppWaveShl1>(v));
+  } else {
+    const int src = lane + OFF * kDYCols;
+    return dshfl(v, (src >= 0 && src < kDW) ? src : lane);
+  }
+}
 }  // namespace
 
 // The nodal update of node p (along index a, across index t) from its
@@ -314,7 +332,6 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   // velocity / gradient arithmetic: FP32 in the mixed form; masses, momenta
   // and coordinates stay FP64
   using T = typename std::conditional<kMixed<R>, float, double>::type;
-  constexpr int S = XD ? 1 : kDYCols;  // lane distance of the along-sweep neighbour
   const Opts& o = kp.o;
   const auto& f = kp.f;
   const long nx = o.nx, ny = o.ny;
@@ -366,13 +383,14 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   T dpx[7], dpy[7];
   double xl[7];
   bool inS[7];
+  dpx[1] = dnbr<XD, -2>(dpx0, lane), dpy[1] = dnbr<XD, -2>(dpy0, lane), xl[1] = dnbr<XD, -2>(xl0, lane);
+  dpx[2] = dnbr<XD, -1>(dpx0, lane), dpy[2] = dnbr<XD, -1>(dpy0, lane), xl[2] = dnbr<XD, -1>(xl0, lane);
+  dpx[4] = dnbr<XD, 1>(dpx0, lane), dpy[4] = dnbr<XD, 1>(dpy0, lane), xl[4] = dnbr<XD, 1>(xl0, lane);
+  dpx[5] = dnbr<XD, 2>(dpx0, lane), dpy[5] = dnbr<XD, 2>(dpy0, lane), xl[5] = dnbr<XD, 2>(xl0, lane);
 #pragma unroll
   for (int d = 1; d <= 5; ++d) {
-    const int off = d - 3;
-    const int src = lane + off * S;
-    const int srcC = (src >= 0 && src < kDW) ? src : lane;
-    dpx[d] = dshfl(dpx0, srcC), dpy[d] = dshfl(dpy0, srcC), xl[d] = dshfl(xl0, srcC);
-    const long b = a + off;
+    if (d == 3) continue;
+    const long b = a + (d - 3);
     inS[d] = (b >= 0 && b <= na);
     if (!inS[d]) dpx[d] = dpy[d] = T(0.0), xl[d] = 0.0;
   }
@@ -384,13 +402,12 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   // own limited nodal gradient (UtilesRemap-Impl.h:8-97).  Each lane forms
   // the divided differences towards its minus neighbour; those towards the
   // plus neighbour are the next lane's (the same operands).
-  const int lmS = (lane - S >= 0) ? lane - S : lane, lpS = (lane + S < kDW) ? lane + S : lane;
   T gl0 = T(0.0), gl1 = T(0.0);
   if (o.order > 1) {
     gl0 = (dpx[3] - dpx[2]) / T(xl[3] - xl[2]);
     gl1 = (dpy[3] - dpy[2]) / T(xl[3] - xl[2]);
   }
-  const T gr0 = dshfl(gl0, lpS), gr1 = dshfl(gl1, lpS);
+  const T gr0 = dnbr<XD, 1>(gl0, lane), gr1 = dnbr<XD, 1>(gl1, lane);
   T g0[2] = {T(0.0), T(0.0)};
   if (in && o.order > 1 && inS[2] && inS[4]) {
     const int d = 3;
@@ -408,8 +425,8 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
     g0[1] = sw::limitedGradient<T>(o.limiter, gr1, gl1, dpy[d + 1], dpy[d - 1], h0, hp, hm);
   }
   T gM[2], gP[2];
-  gM[0] = dshfl(g0[0], lmS), gM[1] = dshfl(g0[1], lmS);
-  gP[0] = dshfl(g0[0], lpS), gP[1] = dshfl(g0[1], lpS);
+  gM[0] = dnbr<XD, -1>(g0[0], lane), gM[1] = dnbr<XD, -1>(g0[1], lane);
+  gP[0] = dnbr<XD, 1>(g0[0], lane), gP[1] = dnbr<XD, 1>(g0[1], lane);
   if (!output) return;
   // a neighbour outside the domain has a zero gradient
   if (!inS[2]) gM[0] = gM[1] = T(0.0);
@@ -704,6 +721,27 @@ __global__ void k_node_mass(KPT<R> kp) {
 // Conservation sums with Neumaier compensation: out[0] mass, out[1]
 // internal energy (sum M*e), out[2+k] material masses.  Per-block partial
 // pairs (sum, compensation) are combined on the host.
+// compact1: writes the derived material fields (fraction and mass fraction
+// 1, the cell's mass, density, energy, pressure and sound speed) of the
+// cells whose envx flag is clear into the material arrays of the current
+// parity, so that every cell's material fields are readable there; the
+// flags are left as they are.
+template <class R>
+__global__ __launch_bounds__(256) void k_env_expand(KPT<R> kp) {
+  const auto& f = kp.f;
+  const int s = kp.cur;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < kp.ncell; c += (long)gridDim.x * blockDim.x) {
+    if (f.envx[c]) continue;
+    f.f_k[0][c] = 1.0;
+    f.mu_k[0][c] = R(1.0);
+    f.M_k[0][c] = f.M[c];
+    f.rho_k[s][0][c] = f.rho[s][c];
+    f.e_k[s][0][c] = f.e[s][c];
+    f.p_k[s][0][c] = f.p[s][c];
+    f.c_k[s][0][c] = f.c[s][c];
+  }
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
   const auto& f = kp.f;
@@ -721,7 +759,9 @@ __global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
     const double M = f.M[c];
     add(0, M);
     add(1, M * (double)f.e[kp.cur][c]);
-    for (int k = 0; k < m; ++k) add(2 + k, f.M_k[k][c]);
+    // (compact1: the material mass is the cell mass unless stored)
+    const bool derived = kp.o.compact1 && !f.envx[c];
+    for (int k = 0; k < m; ++k) add(2 + k, derived ? M : f.M_k[k][c]);
   }
   __shared__ double sh[256];
   for (int q = 0; q < nq; ++q) {
@@ -764,6 +804,7 @@ __global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
   template __global__ void k_remap_final<R, 3>(KPT<R>);                \
   template __global__ void k_node_mass<R>(KPT<R>);                     \
   template __global__ void k_conservation<R>(KPT<R>, double*);         \
+  template __global__ void k_env_expand<R>(KPT<R>);                     \
   template __global__ void k_dual_tile<R>(KPT<R>);
 MHY_REMAP_INST(double)
 MHY_REMAP_INST(float)
