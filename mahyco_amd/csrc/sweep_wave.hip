@@ -53,6 +53,10 @@ template <class R, bool PP>
 constexpr int kYWaves = (kMixed<R> && !PP) ? 8 : 4;
 constexpr int kYRows = kWave / kYCols;
 constexpr int kYOut = kYRows - 2 * kHalo;  // rows updated per wave along y
+// the fused Lagrange sweeps that park the remap vector in LDS and are
+// compiled for 5 waves per SIMD: one material, FP64, limited slopes
+template <class R, int L, bool PP>
+constexpr bool kParkLag = !kMixed<R> && L == 1 && !PP;
 
 // end-of-Lagrange node position (updatePosition, PhaseLagrange.cc:203-208)
 template <class R>
@@ -204,11 +208,14 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
   const bool pb = PP && o.methode == 2;
 
   // LAG: the cell's remap vector at the sweep start, formed by the Lagrange
-  // update (no copy in HBM).  It is parked in LDS, one slot per thread, from
-  // the update to the end of the sweep, so that it holds no registers through
-  // the slope computation; each thread reads back only its own slots.
+  // update (no copy in HBM), kept for the cell's own update at the end.  The
+  // one-material FP64 sweeps park it in LDS, one slot per thread, so that it
+  // holds no registers through the slope computation (each thread reads back
+  // only its own slots); the others keep it in registers.
+  constexpr bool kPark = LAG && kParkLag<R, L, PP>;
   constexpr int kThreads = XD ? 256 : 64 * kYWaves<R, PP>;
-  __shared__ double uPark[LAG ? NC : 1][LAG ? kThreads : 1];
+  __shared__ double uPark[kPark ? NC : 1][kPark ? kThreads : 1];
+  double uOwn[NC];
   // one cell of this lane's line at along index b: Phi, centroid, width, flags
   auto loadCell = [&](long b, T* ph, double& x, T& h, int& flg) {
     const long jb = XD ? t : b, ib = XD ? b : t;
@@ -222,7 +229,11 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
       LagCell<M> lc;
       lagrangeCellCompute<R, M>(kp, cb, jb, ib, output, lc);
 #pragma unroll
-      for (int q = 0; q < NC; ++q) u[q] = uPark[q][threadIdx.x] = lc.U[q];
+      for (int q = 0; q < NC; ++q) {
+        u[q] = lc.U[q];
+        if constexpr (kPark) uPark[q][threadIdx.x] = lc.U[q];
+        else uOwn[q] = lc.U[q];
+      }
       V = lc.V, qv = lc.q;
       x = XD ? lc.xc.x : lc.xc.y;
       if (output && o.methode == 1) f.rL[cb] = lc.rho;  // A2 dual fluxes
@@ -371,8 +382,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
       for (int k = 0; k < M; ++k) dfv[k] = formed ? (double)Dual[k] * w : 0.0;
     }
   }
-  double uOwn[NC];
-  if constexpr (LAG) {
+  if constexpr (kPark) {
 #pragma unroll
     for (int q = 0; q < NC; ++q) uOwn[q] = uPark[q][threadIdx.x];
   }
@@ -416,7 +426,7 @@ constexpr int kMinWavesBatch = (M < 3) ? kMinWaves<R, PP, XD> : 1;
 template <class R, int L, bool PP, bool XD>
 constexpr int kMinWavesPlain = (!kMixed<R> && L == 1 && !PP) ? 5 : kMinWaves<R, PP, XD>;
 template <class R, int L, bool PP, bool XD>
-constexpr int kMinWavesLag = (!kMixed<R> && L == 1 && !PP) ? 5 : kMinWaves<R, PP, XD>;
+constexpr int kMinWavesLag = kParkLag<R, L, PP> ? 5 : kMinWaves<R, PP, XD>;
 
 template <class R, int M, bool PP, bool XD>
 __global__ __launch_bounds__((XD ? 256 : 64 * kYWaves<R, PP>))
