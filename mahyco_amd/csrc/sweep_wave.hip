@@ -321,15 +321,29 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     const R* uu = XD ? f.ux[kp.nxt] : f.uy[kp.nxt];
     return 0.0 + ((0.0 + (double)uu[n1]) + (double)uu[n2]) * 0.5;
   };
-  const double vm = in ? fnv(a) : 0.0, vp = in ? fnv(a + 1) : 0.0;
-  // Euler lengths of the minus / plus faces (Init.cc:528-533)
+  const double vm = in ? fnv(a) : 0.0;
+  double vp;
+  if constexpr (XD) {
+    // along x a lane's plus face is the next lane's minus face (the same
+    // nodes, so the same bits), taken by a DPP move where that cell is in the
+    // domain; the wave's last lane keeps its own value there, unused: it is
+    // a halo lane, and only its own update reads its plus face
+    const double vn = nbrP<XD>(vm, lp);
+    vp = (in && a + 1 < na) ? vn : (in ? fnv(a + 1) : 0.0);
+  } else {
+    vp = in ? fnv(a + 1) : 0.0;
+  }
+  // Euler lengths of the minus / plus faces (Init.cc:528-533).  A face's two
+  // nodes share their along-sweep coordinate, whose difference is exactly 0,
+  // so the length depends on the across index only: the plus face's is the
+  // minus face's, bit for bit.
   auto faceLen = [&](long b) {
     D2 e1, e2;
     if (XD) e1 = eulerNode(o, j, b), e2 = eulerNode(o, j + 1, b);
     else e1 = eulerNode(o, b, i), e2 = eulerNode(o, b, i + 1);
     return d2norm(d2sub(e2, e1));
   };
-  const double Lm = faceLen(a), Lp = faceLen(a + 1);
+  const double Lm = faceLen(a), Lp = Lm;
   const bool hasM = inM, hasP = inP;
 
   T phiM[NC], phiP[NC];
