@@ -214,6 +214,19 @@ struct EucFields {
   double *outvx, *outvy;
 };
 
+// Element idx of a device array addressed by a 32-bit byte offset: the load
+// or store takes the array base from scalar registers and the offset from one
+// 32-bit vector register, instead of a 64-bit address per array.  Every array
+// of a context is below 4 GiB (checked when the context is created).
+template <class V>
+__device__ __forceinline__ V& at(V* base, long idx) {
+  return *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + (unsigned)idx * (unsigned)sizeof(V));
+}
+template <class V>
+__device__ __forceinline__ const V& at(const V* base, long idx) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + (unsigned)idx * (unsigned)sizeof(V));
+}
+
 // bitwise equality (distinguishes signed zeros, never true for a NaN pattern
 // compared with another)
 __device__ __forceinline__ bool sameBits(double a, double b) {

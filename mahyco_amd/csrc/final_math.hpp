@@ -114,19 +114,19 @@ __device__ __forceinline__ void finalCompute(const KPT<R>& kp, long c, const dou
       pk[k] = (g - 1.0) * rk[k] * ek[k];
       ck[k] = kMixed<R> ? (double)sqrtf(float(g * (g - 1.0) * ek[k])) : sqrt(g * (g - 1.0) * ek[k]);
     } else if (o.eos[k] == MHY_EOS_VOID) {
-      pk[k] = (double)f.p_k[cur][k][c];
-      ck[k] = (double)f.c_k[cur][k][c];
-      ek[k] = (double)f.e_k[cur][k][c];
+      pk[k] = (double)at(f.p_k[cur][k], c);
+      ck[k] = (double)at(f.c_k[cur][k], c);
+      ek[k] = (double)at(f.e_k[cur][k], c);
       e1 = 0.0;
     } else {
-      pk[k] = (double)f.p_k[nxt][k][c];
-      ck[k] = (double)f.c_k[nxt][k][c];
+      pk[k] = (double)at(f.p_k[nxt][k], c);
+      ck[k] = (double)at(f.c_k[nxt][k], c);
     }
   }
   // computePressionMoyenne
   double p1 = 0.0, c1 = 0.0;
   const bool needStale = !(rho1 > 0.);
-  if (needStale) c1 = (double)f.c[nxt][c];
+  if (needStale) c1 = (double)at(f.c[nxt], c);
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     p1 += fr[k] * pk[k];
@@ -163,14 +163,14 @@ template <class R, int M>
 __device__ __forceinline__ void finalStore(const KPT<R>& kp, long c, const FinalOut<R, M>& out) {
   const auto& f = kp.f;
   const int nxt = kp.nxt;
-  f.rho[nxt][c] = out.rho;
-  f.e[nxt][c] = out.e;
-  f.q[nxt][c] = out.q;
-  f.p[nxt][c] = out.p;
-  f.c[nxt][c] = out.c;
-  f.M[c] = out.Mc;
-  f.mixte[c] = out.mixte;
-  f.pure[c] = out.pure;
+  at(f.rho[nxt], c) = out.rho;
+  at(f.e[nxt], c) = out.e;
+  at(f.q[nxt], c) = out.q;
+  at(f.p[nxt], c) = out.p;
+  at(f.c[nxt], c) = out.c;
+  at(f.M, c) = out.Mc;
+  at(f.mixte, c) = out.mixte;
+  at(f.pure, c) = out.pure;
   if constexpr (M == 1) {
     if (kp.o.compact1) {
       // the material fields equal 1, 1 and the mean fields bit for bit in
@@ -179,19 +179,19 @@ __device__ __forceinline__ void finalStore(const KPT<R>& kp, long c, const Final
       const bool derived = sameBits(out.fr[0], 1.0) && sameBits(out.mu[0], R(1.0)) && sameBits(out.Mk[0], out.Mc) &&
                            sameBits(out.rk[0], out.rho) && sameBits(out.ek[0], out.e) &&
                            sameBits(out.pk[0], out.p) && sameBits(out.ck[0], out.c);
-      f.envx[c] = derived ? 0 : 1;
+      at(f.envx, c) = derived ? 0 : 1;
       if (derived) return;
     }
   }
 #pragma unroll
   for (int k = 0; k < M; ++k) {
-    f.f_k[k][c] = out.fr[k];
-    f.mu_k[k][c] = out.mu[k];
-    f.M_k[k][c] = out.Mk[k];
-    f.rho_k[nxt][k][c] = out.rk[k];
-    f.e_k[nxt][k][c] = out.ek[k];
-    f.p_k[nxt][k][c] = out.pk[k];
-    f.c_k[nxt][k][c] = out.ck[k];
+    at(f.f_k[k], c) = out.fr[k];
+    at(f.mu_k[k], c) = out.mu[k];
+    at(f.M_k[k], c) = out.Mk[k];
+    at(f.rho_k[nxt][k], c) = out.rk[k];
+    at(f.e_k[nxt][k], c) = out.ek[k];
+    at(f.p_k[nxt][k], c) = out.pk[k];
+    at(f.c_k[nxt][k], c) = out.ck[k];
   }
 }
 

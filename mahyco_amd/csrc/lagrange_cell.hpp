@@ -33,10 +33,10 @@ __device__ __forceinline__ void lagNodes(const KPT<R>& kp, double dt1, long j, l
   for (int s = 0; s < 4; ++s) {
     const long p = jj[s] * (nx + 1) + ii[s];
     if (o.mode == MODE_LAGRANGE) {
-      X[s] = {f.X[kp.nxt][p], f.Y[kp.nxt][p]};
+      X[s] = {at(f.X[kp.nxt], p), at(f.Y[kp.nxt], p)};
     } else {
       const D2 e = eulerNode(o, jj[s], ii[s]);
-      X[s] = {e.x + (double)f.ux[kp.nxt][p] * dt1, e.y + (double)f.uy[kp.nxt][p] * dt1};
+      X[s] = {e.x + (double)at(f.ux[kp.nxt], p) * dt1, e.y + (double)at(f.uy[kp.nxt], p) * dt1};
     }
   }
 }
@@ -105,26 +105,26 @@ __device__ __forceinline__ bool lagrangeCellCompute(const KPT<R>& kp, long c, lo
   // the cell's step-start words, loaded up front: their uses below are
   // conditional, and unconditional loads can all be in flight together
   double fk[NMAT], Mk0[NMAT], rk0[NMAT], qk0[NMAT], ek0[NMAT], pk0[NMAT], mu0[NMAT];
-  const double rhoS = (double)f.rho[cur][c], divuS = (double)f.divu[cur][c], cS = (double)f.c[cur][c];
-  const double vol0 = f.vol[cur][c], Mc0 = f.M[c];
+  const double rhoS = (double)at(f.rho[cur], c), divuS = (double)at(f.divu[cur], c), cS = (double)at(f.c[cur], c);
+  const double vol0 = at(f.vol[cur], c), Mc0 = at(f.M, c);
   // one material (compact1): fraction and mass fraction 1, the material
   // mass, density, energy and pressure those of the cell, unless the final
   // remap stored them (envx)
-  const bool derived = (m == 1) && o.compact1 && !f.envx[c];
+  const bool derived = (m == 1) && o.compact1 && !at(f.envx, c);
   if (derived) {
     fk[0] = 1.0, mu0[0] = 1.0, Mk0[0] = Mc0, rk0[0] = rhoS;
-    ek0[0] = (double)f.e[cur][c], pk0[0] = (double)f.p[cur][c];
-    qk0[0] = (double)f.q_k[cur][0][c];
+    ek0[0] = (double)at(f.e[cur], c), pk0[0] = (double)at(f.p[cur], c);
+    qk0[0] = (double)at(f.q_k[cur][0], c);
   } else {
 #pragma unroll
     for (int k = 0; k < m; ++k) {
-      fk[k] = f.f_k[k][c];
-      Mk0[k] = f.M_k[k][c];
-      rk0[k] = (double)f.rho_k[cur][k][c];
-      qk0[k] = (double)f.q_k[cur][k][c];
-      ek0[k] = (double)f.e_k[cur][k][c];
-      pk0[k] = (double)f.p_k[cur][k][c];
-      mu0[k] = (double)f.mu_k[k][c];
+      fk[k] = at(f.f_k[k], c);
+      Mk0[k] = at(f.M_k[k], c);
+      rk0[k] = (double)at(f.rho_k[cur][k], c);
+      qk0[k] = (double)at(f.q_k[cur][k], c);
+      ek0[k] = (double)at(f.e_k[cur][k], c);
+      pk0[k] = (double)at(f.p_k[cur][k], c);
+      mu0[k] = (double)at(f.mu_k[k], c);
     }
   }
   D2 X[4];
@@ -138,15 +138,15 @@ __device__ __forceinline__ bool lagrangeCellCompute(const KPT<R>& kp, long c, lo
     // absent materials keep the stale buffer value in the reference; their
     // volume fraction is exactly zero, so the value never contributes
     rk1[k] = (fk[k] > o.threshold) ? Mk0[k] / (fk[k] * V)
-                                   : (o.mode == MODE_FULL ? 0.0 : (double)f.rho_k[nxt][k][c]);
+                                   : (o.mode == MODE_FULL ? 0.0 : (double)at(f.rho_k[nxt][k], c));
     rho1 += fk[k] * rk1[k];
   }
   const bool periodicCell = o.periodic_right && (i == 0 || i == nx - 1);
   if (periodicCell) {
     const long L = j * nx;  // updatePeriodicBoundaryConditions
-    rho1 = (double)f.rho0[L];
+    rho1 = (double)at(f.rho0, L);
 #pragma unroll
-    for (int k = 0; k < m; ++k) rk1[k] = f.rho_k0[k][L];
+    for (int k = 0; k < m; ++k) rk1[k] = at(f.rho_k0[k], L);
   }
 
   double q1 = 0.0, e1 = 0.0;
@@ -196,18 +196,18 @@ __device__ __forceinline__ bool lagrangeCellCompute(const KPT<R>& kp, long c, lo
         ck1[k] = sqrt(g * (g - 1.0) * ek1[k]);
       } else if (o.eos[k] == MHY_EOS_VOID) {
         pk1[k] = pk0[k];
-        ck1[k] = f.c_k[cur][k][c];
+        ck1[k] = at(f.c_k[cur][k], c);
         ek1[k] = ek0[k];
         e1 = 0.0;
       } else {
-        pk1[k] = f.p_k[nxt][k][c];
-        ck1[k] = f.c_k[nxt][k][c];
+        pk1[k] = at(f.p_k[nxt][k], c);
+        ck1[k] = at(f.c_k[nxt][k], c);
       }
     }
     // computePressionMoyenne (PhaseLagrange.cc:464-485)
     double p1 = 0.0, c1 = 0.0;
     const bool needStale = !(rho1 > 0.);
-    if (needStale) c1 = (double)f.c[nxt][c];
+    if (needStale) c1 = (double)at(f.c[nxt], c);
 #pragma unroll
     for (int k = 0; k < m; ++k) {
       p1 += fk[k] * pk1[k];
@@ -218,25 +218,25 @@ __device__ __forceinline__ bool lagrangeCellCompute(const KPT<R>& kp, long c, lo
     for (int k = 0; k < m; ++k)
       if (o.eos[k] == MHY_EOS_VOID) e1 += mu0[k] * ek1[k];
     if (store) {
-      f.divu[nxt][c] = div1;
+      at(f.divu[nxt], c) = div1;
 #pragma unroll
-      for (int k = 0; k < m; ++k) f.q_k[nxt][k][c] = qk1[k];
+      for (int k = 0; k < m; ++k) at(f.q_k[nxt][k], c) = qk1[k];
     }
     if (o.mode == MODE_LAGRANGE) {
       if (store) {
-        f.rho[nxt][c] = rho1;
-        f.q[nxt][c] = q1;
-        f.e[nxt][c] = e1;
-        f.p[nxt][c] = p1;
-        f.c[nxt][c] = c1;
+        at(f.rho[nxt], c) = rho1;
+        at(f.q[nxt], c) = q1;
+        at(f.e[nxt], c) = e1;
+        at(f.p[nxt], c) = p1;
+        at(f.c[nxt], c) = c1;
 #pragma unroll
         for (int k = 0; k < m; ++k) {
-          f.rho_k[nxt][k][c] = rk1[k];
-          f.e_k[nxt][k][c] = ek1[k];
-          f.p_k[nxt][k][c] = pk1[k];
-          f.c_k[nxt][k][c] = ck1[k];
+          at(f.rho_k[nxt][k], c) = rk1[k];
+          at(f.e_k[nxt][k], c) = ek1[k];
+          at(f.p_k[nxt][k], c) = pk1[k];
+          at(f.c_k[nxt][k], c) = ck1[k];
         }
-        f.vol[nxt][c] = V;  // node_cellvolume_n swap (VnrRemap.cc:217-221)
+        at(f.vol[nxt], c) = V;  // node_cellvolume_n swap (VnrRemap.cc:217-221)
       }
       return false;
     }
@@ -244,8 +244,8 @@ __device__ __forceinline__ bool lagrangeCellCompute(const KPT<R>& kp, long c, lo
     // advection: the energy swap of VnrRemap.cc:170 makes the remap read the
     // current energy; the pseudo buffer of the n+1 slot is read as left by
     // the previous remap of that slot
-    e1 = (double)f.e[cur][c];
-    q1 = (double)f.q[nxt][c];
+    e1 = (double)at(f.e[cur], c);
+    q1 = (double)at(f.q[nxt], c);
   }
   // computeVariablesForRemap (PrepareRemap.cc:96-176), live components only
 #pragma unroll

@@ -39,7 +39,7 @@ __device__ __forceinline__ void sweepUnew(const KPT<R>& kp, long c, long i, long
   const auto& f = kp.f;
   const long nx = o.nx, ny = o.ny;
   double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
-  auto uOld = [&](int q) { return Uown ? Uown[q] : Uin[q][c]; };
+  auto uOld = [&](int q) { return Uown ? Uown[q] : at(Uin[q], c); };
   const double dtn = f.sc->dt[kp.cur];
   // computeUremap (PhaseRemap1.cc:468-503): a cell receives -G through its
   // minus face and +G through its plus face
@@ -128,12 +128,12 @@ __device__ __forceinline__ void sweepUpdateCell(const KPT<R>& kp, long c, long i
   if (!stored) {
     double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
 #pragma unroll
-    for (int q = 0; q < NC; ++q) Uout[q][c] = Unew[q];
+    for (int q = 0; q < NC; ++q) at(Uout[q], c) = Unew[q];
   }
   if (kp.o.methode == 0 || kp.o.methode == 1) {
     const long fm = XD ? j * (kp.o.nx + 1) + a : a * kp.o.nx + i;
 #pragma unroll
-    for (int k = 0; k < M; ++k) f.G[k][fm] = Gd[k];
+    for (int k = 0; k < M; ++k) at(f.G[k], fm) = Gd[k];
   }
 }
 
@@ -159,11 +159,11 @@ __device__ __forceinline__ void storeCellOut(const KPT<R>& kp, long c, long fm, 
   } else {
     double* const* Uout = (kp.sweep == 1) ? f.U1 : f.U2;
 #pragma unroll
-    for (int q = 0; q < kNC<L>; ++q) Uout[q][c] = out.U[q];
+    for (int q = 0; q < kNC<L>; ++q) at(Uout[q], c) = out.U[q];
   }
   if (withG) {
 #pragma unroll
-    for (int k = 0; k < kMat<L>; ++k) f.G[k][fm] = out.G[k];
+    for (int k = 0; k < kMat<L>; ++k) at(f.G[k], fm) = out.G[k];
   }
 }
 

@@ -63,7 +63,7 @@ template <class R>
 __device__ __forceinline__ D2 lagNodeW(const KPT<R>& kp, double dt1, long j, long i) {
   const long p = j * (kp.o.nx + 1) + i;
   const D2 e = eulerNode(kp.o, j, i);
-  return {e.x + (double)kp.f.ux[kp.nxt][p] * dt1, e.y + (double)kp.f.uy[kp.nxt][p] * dt1};
+  return {e.x + (double)at(kp.f.ux[kp.nxt], p) * dt1, e.y + (double)at(kp.f.uy[kp.nxt], p) * dt1};
 }
 
 // width across the sweep faces (HvLagrange, PhaseRemap1.cc:57-71, :104-119)
@@ -147,7 +147,7 @@ __device__ __forceinline__ void sweepYStaged(const KPT<R>& kp, long c, long i, l
       storeCellOut<R, L, FUSE>(kp, cc, cc, out, withG);  // the minus face of a y-cell has the cell's index
       if (pb) {
 #pragma unroll
-        for (int k = 0; k < M; ++k) f.DF[k][cc] = out.DF[k];
+        for (int k = 0; k < M; ++k) at(f.DF[k], cc) = out.DF[k];
       }
     }
   }
@@ -236,31 +236,31 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
       }
       V = lc.V, qv = lc.q;
       x = XD ? lc.xc.x : lc.xc.y;
-      if (output && o.methode == 1) f.rL[cb] = lc.rho;  // A2 dual fluxes
-      if (output && PP) f.qL[cb] = lc.q;                // PenteBorne Phi of the second sweep
-      if (output && kp.geomLoad) f.VL[cb] = lc.V, f.XcL[cb] = lc.xc.x, f.YcL[cb] = lc.xc.y;
+      if (output && o.methode == 1) at(f.rL, cb) = lc.rho;  // A2 dual fluxes
+      if (output && PP) at(f.qL, cb) = lc.q;                // PenteBorne Phi of the second sweep
+      if (output && kp.geomLoad) at(f.VL, cb) = lc.V, at(f.XcL, cb) = lc.xc.x, at(f.YcL, cb) = lc.xc.y;
     } else {
       // (the VNR PenteBorne Phi takes the pseudo from q_L, not from U)
       const int nload = (PP && !kEuc<L>) ? 3 * M : NC;
 #pragma unroll
-      for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cb] : 0.0;
+      for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? at(Uin[q], cb) : 0.0;
       // the Lagrange volume and centroid: stored by the cell update, or
       // formed again from the end-of-Lagrange node positions (bitwise the
       // values the cell update formed)
       if (kEuc<L> || kp.geomLoad) {  // (Eucclhyd: always stored)
-        V = f.VL[cb];
-        x = XD ? f.XcL[cb] : f.YcL[cb];
+        V = at(f.VL, cb);
+        x = XD ? at(f.XcL, cb) : at(f.YcL, cb);
       } else {
         D2 xcv;
         lagGeometry(kp, dt1, jb, ib, V, xcv);
         x = XD ? xcv.x : xcv.y;
       }
-      if (PP && !kEuc<L>) qv = f.qL[cb];
+      if (PP && !kEuc<L>) qv = at(f.qL, cb);
     }
     cellPhi<L, PP, T>(u, PP ? 1.0 : V, qv, ph);
     h = needH ? T(widthW<XD>(kp, dt1, jb, ib)) : T(0.0);
     flg = 0;
-    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<L, T>(ph);
+    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(at(f.mixte, cb), at(f.pure, cb)) : packedFlags<L, T>(ph);
   };
   T phi[NC], H = T(0.0);
   double xc = 0.0;
@@ -319,7 +319,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     if (XD) n1 = j * (nx + 1) + b, n2 = (j + 1) * (nx + 1) + b;
     else n1 = b * (nx + 1) + i, n2 = b * (nx + 1) + i + 1;
     const R* uu = XD ? f.ux[kp.nxt] : f.uy[kp.nxt];
-    return 0.0 + ((0.0 + (double)uu[n1]) + (double)uu[n2]) * 0.5;
+    return 0.0 + ((0.0 + (double)at(uu, n1)) + (double)at(uu, n2)) * 0.5;
   };
   const double vm = in ? fnv(a) : 0.0;
   double vp;
@@ -404,7 +404,7 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
     if (!output) return;
     if (pb) {
 #pragma unroll
-      for (int k = 0; k < M; ++k) f.DF[k][c] = dfv[k];
+      for (int k = 0; k < M; ++k) at(f.DF[k], c) = dfv[k];
     }
     sweepUpdateCell<R, L, PP, XD, T>(kp, c, i, j, a, hasM, hasP, vm, vp, Lm, Lp, phiM, phiP, LAG ? uOwn : nullptr);
   } else {
