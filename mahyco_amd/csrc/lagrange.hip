@@ -300,12 +300,12 @@ __device__ __forceinline__ void lagrangeCellBody(const KPT<R>& kp, unsigned b) {
   if (!lagrangeCellCompute<R, M>(kp, c, j, i, true, lc)) return;
   const auto& f = kp.f;
 #pragma unroll
-  for (int q = 0; q < 3 * M + 1; ++q) f.UL[q][c] = lc.U[q];
-  f.VL[c] = lc.V;
-  if (kp.o.methode == 1) f.rL[c] = lc.rho;
-  if (kp.o.pp == 1) f.qL[c] = lc.q;
-  f.XcL[c] = lc.xc.x;
-  f.YcL[c] = lc.xc.y;
+  for (int q = 0; q < 3 * M + 1; ++q) at(f.UL[q], c) = lc.U[q];
+  at(f.VL, c) = lc.V;
+  if (kp.o.methode == 1) at(f.rL, c) = lc.rho;
+  if (kp.o.pp == 1) at(f.qL, c) = lc.q;
+  at(f.XcL, c) = lc.xc.x;
+  at(f.YcL, c) = lc.xc.y;
 }
 
 template <class R, int M>

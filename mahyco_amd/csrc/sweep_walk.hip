@@ -30,7 +30,7 @@ template <class R>
 __device__ __forceinline__ D2 lagNodeK(const KPT<R>& kp, double dt1, long j, long i) {
   const long p = j * (kp.o.nx + 1) + i;
   const D2 e = eulerNode(kp.o, j, i);
-  return {e.x + (double)kp.f.ux[kp.nxt][p] * dt1, e.y + (double)kp.f.uy[kp.nxt][p] * dt1};
+  return {e.x + (double)at(kp.f.ux[kp.nxt], p) * dt1, e.y + (double)at(kp.f.uy[kp.nxt], p) * dt1};
 }
 
 // width across the y-sweep faces (HvLagrange, PhaseRemap2.cc): right then
@@ -76,17 +76,17 @@ __device__ __forceinline__ void sweepWalkYBody(const KPT<R>& kp, unsigned b) {
     double u[NC];
     const int nload = PP ? 3 * M : NC;
 #pragma unroll
-    for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? Uin[q][cb] : 0.0;
-    cellPhi<M, PP, T>(u, PP ? 1.0 : f.VL[cb], PP ? f.qL[cb] : 0.0, ph);
-    x = f.YcL[cb];
+    for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? at(Uin[q], cb) : 0.0;
+    cellPhi<M, PP, T>(u, PP ? 1.0 : at(f.VL, cb), PP ? at(f.qL, cb) : 0.0, ph);
+    x = at(f.YcL, cb);
     h = needH ? T(widthY(kp, dt1, r, i)) : T(0.0);
     flg = 0;
-    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(f.mixte[cb], f.pure[cb]) : packedFlags<M, T>(ph);
+    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(at(f.mixte, cb), at(f.pure, cb)) : packedFlags<M, T>(ph);
   };
   // normal velocity of face r, between rows r-1 and r (PrepareRemap.cc:50-62)
   auto fnv = [&](long r) {
     const long n1 = r * (nx + 1) + i, n2 = n1 + 1;
-    return 0.0 + ((0.0 + (double)uy[n1]) + (double)uy[n2]) * 0.5;
+    return 0.0 + ((0.0 + (double)at(uy, n1)) + (double)at(uy, n2)) * 0.5;
   };
   // Euler length of face r (Init.cc:528-533)
   auto faceLen = [&](long r) { return d2norm(d2sub(eulerNode(o, r, i + 1), eulerNode(o, r, i))); };
@@ -184,7 +184,7 @@ __device__ __forceinline__ void sweepWalkYBody(const KPT<R>& kp, unsigned b) {
       if (pb) {
         const double w = (Lm + Lp) * 0.5;
 #pragma unroll
-        for (int k2 = 0; k2 < M; ++k2) f.DF[k2][c] = formed ? (double)dualA[k2] * w : 0.0;
+        for (int k2 = 0; k2 < M; ++k2) at(f.DF[k2], c) = formed ? (double)dualA[k2] * w : 0.0;
       }
       sweepUpdateCell<R, M, PP, false, T>(kp, c, i, rA, rA, hasM, hasP, vA, vB, Lm, Lp, phiM, phiP);
     }
