@@ -610,7 +610,7 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
           [&] { launchR(x, k_dual<double, true>, k_dual<float, true>, gridFor(x->nnode), dim3(kBlock), kp); });
   } else if (x->dualTileY) {
     timed(x, K_DUAL_Y, [&] {
-      launchR(x, k_dual_tile<double>, k_dual_tile<float>, dualTileGrid(x->o.nx, x->o.ny), dim3(kBlock), kp);
+      launchR(x, k_dual_tile<double>, k_dual_tile<float>, dualTileGrid(x->o.nx, x->o.ny), dim3(dualTileBlock()), kp);
     });
   } else if (x->dualWalkY) {
     timed(x, K_DUAL_Y, [&] {

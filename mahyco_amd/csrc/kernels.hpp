@@ -83,6 +83,7 @@ __global__ void k_dual_walk(KPT<R> kp);
 template <class R>
 __global__ void k_dual_tile(KPT<R> kp);
 dim3 dualTileGrid(long nx, long ny);
+unsigned dualTileBlock();
 dim3 dualWalkGrid(long nx, long ny);
 unsigned dualWaveBlock(bool xd, bool mixed);
 template <class R, int M>

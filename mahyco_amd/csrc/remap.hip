@@ -551,9 +551,10 @@ dim3 dualWalkGrid(long nx, long ny) {
 // the inner rows; every value comes from the same operands as in the
 // wave-strip kernel, so results are bitwise the same.
 namespace {
+constexpr int kDTThreads = 256;                 // threads per tile (512 and 1024: no faster)
 constexpr int kDTCols = 16;                     // node columns per tile
-constexpr int kDTRowsPer = 4;                   // rows per thread
-constexpr int kDTRows = 256 / kDTCols * kDTRowsPer;  // rows per tile (64)
+constexpr int kDTRows = 64;                     // node rows per tile
+constexpr int kDTRowsPer = kDTRows * kDTCols / kDTThreads;  // rows per thread
 constexpr int kDTOut = kDTRows - 6;             // rows updated (58)
 }  // namespace
 
@@ -573,7 +574,7 @@ __device__ __forceinline__ void dualTileBody(const KPT<R>& kp, unsigned b) {
   __shared__ T sdx[kDTRows][kDTCols], sdy[kDTRows][kDTCols], sgx[kDTRows][kDTCols], sgy[kDTRows][kDTCols];
   __shared__ double sxl[kDTRows][kDTCols];
   // tile row of the k-th row of this thread: 16 consecutive rows per pass
-  auto tileRow = [&](int k) { return rg + k * (256 / kDTCols); };
+  auto tileRow = [&](int k) { return rg + k * (kDTThreads / kDTCols); };
   auto inRow = [&](int L) { return (r0 + L >= 0) && (r0 + L <= na); };
   // own DualPhi and Lagrange coordinate of the tile's nodes
 #pragma unroll
@@ -649,11 +650,13 @@ __device__ __forceinline__ void dualTileBody(const KPT<R>& kp, unsigned b) {
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_dual_tile(KPT<R> kp) {
+__global__ __launch_bounds__(kDTThreads) void k_dual_tile(KPT<R> kp) {
   // neighbouring column groups share the cache lines of the face fluxes
   // left of their first column: one XCD (one L2) takes a contiguous range
   dualTileBody<R>(kp, xcdBlock());
 }
+
+unsigned dualTileBlock() { return kDTThreads; }
 
 dim3 dualTileGrid(long nx, long ny) {
   const long gx = (nx + 1 + kDTCols - 1) / kDTCols, gy = (ny + 1 + kDTOut - 1) / kDTOut;
