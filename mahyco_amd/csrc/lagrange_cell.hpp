@@ -29,14 +29,17 @@ __device__ __forceinline__ void lagNodes(const KPT<R>& kp, double dt1, long j, l
   const auto& f = kp.f;
   const long nx = o.nx;
   const long jj[4] = {j, j, j + 1, j + 1}, ii[4] = {i, i + 1, i + 1, i};
+  // the corners' node indices differ from the first by uniform amounts,
+  // which go into the array bases: one 32-bit offset serves the four loads
+  const long p0 = j * (nx + 1) + i;
+  const long dp[4] = {0, 1, nx + 2, nx + 1};
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    const long p = jj[s] * (nx + 1) + ii[s];
     if (o.mode == MODE_LAGRANGE) {
-      X[s] = {at(f.X[kp.nxt], p), at(f.Y[kp.nxt], p)};
+      X[s] = {at(f.X[kp.nxt] + dp[s], p0), at(f.Y[kp.nxt] + dp[s], p0)};
     } else {
       const D2 e = eulerNode(o, jj[s], ii[s]);
-      X[s] = {e.x + (double)at(f.ux[kp.nxt], p) * dt1, e.y + (double)at(f.uy[kp.nxt], p) * dt1};
+      X[s] = {e.x + (double)at(f.ux[kp.nxt] + dp[s], p0) * dt1, e.y + (double)at(f.uy[kp.nxt] + dp[s], p0) * dt1};
     }
   }
 }

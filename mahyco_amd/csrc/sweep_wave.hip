@@ -315,11 +315,11 @@ __device__ __forceinline__ void sweepWaveBody(const KPT<R>& kp, unsigned b) {
 
   // normal velocities of the minus / plus faces (PrepareRemap.cc:50-62)
   auto fnv = [&](long b) {
-    long n1, n2;
-    if (XD) n1 = j * (nx + 1) + b, n2 = (j + 1) * (nx + 1) + b;
-    else n1 = b * (nx + 1) + i, n2 = b * (nx + 1) + i + 1;
+    // the face's second node is a uniform distance from the first (next row
+    // along x, next column along y): one 32-bit offset serves both loads
+    const long n1 = XD ? j * (nx + 1) + b : b * (nx + 1) + i;
     const R* uu = XD ? f.ux[kp.nxt] : f.uy[kp.nxt];
-    return 0.0 + ((0.0 + (double)at(uu, n1)) + (double)at(uu, n2)) * 0.5;
+    return 0.0 + ((0.0 + (double)at(uu, n1)) + (double)at(uu + (XD ? nx + 1 : 1), n1)) * 0.5;
   };
   const double vm = in ? fnv(a) : 0.0;
   double vp;
