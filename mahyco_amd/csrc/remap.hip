@@ -47,7 +47,7 @@ __device__ __forceinline__ void dualBody(const KPT<R>& kp, unsigned b) {
     }
     const long q = p + (d - 3) * stride;
     if (kp.sweep == 1) {
-      const double ux1 = (double)f.ux[kp.nxt][q], uy1 = (double)f.uy[kp.nxt][q];
+      const double ux1 = (double)at(f.ux[kp.nxt], q), uy1 = (double)at(f.uy[kp.nxt], q);
       dpx[d] = ux1, dpy[d] = uy1;
       const long jq = q / (nx + 1), iq = q - jq * (nx + 1);
       const D2 e = eulerNode(o, jq, iq);
@@ -55,15 +55,15 @@ __device__ __forceinline__ void dualBody(const KPT<R>& kp, unsigned b) {
     } else {
       // sweep 2 overwrites the velocity in place, so the Lagrange
       // coordinates come from the copy sweep 1 kept
-      const double M1 = f.UD1[2][q];
-      dpx[d] = f.UD1[0][q] / M1, dpy[d] = f.UD1[1][q] / M1;
-      xl[d] = f.XL2[q];
+      const double M1 = at(f.UD1[2], q);
+      dpx[d] = at(f.UD1[0], q) / M1, dpy[d] = at(f.UD1[1], q) / M1;
+      xl[d] = at(f.XL2, q);
     }
   }
   if (kp.sweep == 1) {
     // Lagrange coordinate of this node along the second sweep's axis
     const D2 e = eulerNode(o, j, i);
-    f.XL2[p] = XD ? e.y + (double)f.uy[kp.nxt][p] * dt1 : e.x + (double)f.ux[kp.nxt][p] * dt1;
+    at(f.XL2, p) = XD ? e.y + (double)at(f.uy[kp.nxt], p) * dt1 : e.x + (double)at(f.ux[kp.nxt], p) * dt1;
   }
   // divided differences between slots k and k+1 (k = 1..4), each formed
   // once: the plus side of slot d is the minus side of slot d+1
@@ -116,15 +116,17 @@ __device__ __forceinline__ void dualBody(const KPT<R>& kp, unsigned b) {
     // G index of face (along b, across t); cells on both sides of it
     auto gidx = [&](long b, long tt) { return XD ? tt * (nx + 1) + b : b * nx + tt; };
     auto cidx = [&](long b, long tt) { return XD ? tt * nx + b : b * nx + tt; };
+    // the along-sweep neighbour's face / cell: a uniform step, folded into the base
+    const long dA = XD ? 1 : nx;
     for (int k = 0; k < m; ++k) {
       double pk = 0, mk = 0;
       if (o.methode == 0 || o.methode == 1) {
         for (int s = 0; s < nf; ++s) {
           const long tt = tf[s];
-          const double Gf = f.G[k][gidx(a, tt)];
-          double tp = Gf + f.G[k][gidx(a + 1, tt)];
-          double tm = Gf + f.G[k][gidx(a - 1, tt)];
-          if (o.methode == 1) tp = f.rL[cidx(a, tt)] * tp, tm = f.rL[cidx(a - 1, tt)] * tm;
+          const double Gf = at(f.G[k], gidx(a, tt));
+          double tp = Gf + at(f.G[k] + dA, gidx(a, tt));
+          double tm = Gf + at(f.G[k] - dA, gidx(a, tt));
+          if (o.methode == 1) tp = at(f.rL, cidx(a, tt)) * tp, tm = at(f.rL - dA, cidx(a, tt)) * tm;
           pk += tp;
           mk += tm;
         }
@@ -134,8 +136,8 @@ __device__ __forceinline__ void dualBody(const KPT<R>& kp, unsigned b) {
         int nbcell = 0;
         for (int s = 0; s < nf; ++s) {
           const long tt = tf[s];
-          pk += f.DF[k][cidx(a, tt)];
-          mk += f.DF[k][cidx(a - 1, tt)];
+          pk += at(f.DF[k], cidx(a, tt));
+          mk += at(f.DF[k] - dA, cidx(a, tt));
           nbcell++;
         }
         if (nbcell != 0) pk /= nbcell, mk /= nbcell;
@@ -165,20 +167,20 @@ __device__ __forceinline__ void dualBody(const KPT<R>& kp, unsigned b) {
   upwind(false, plus, gP, uvp);
   double U0, U1, U2;
   if (kp.sweep == 1) {
-    const double Mp = f.Mp[p];
-    U0 = Mp * f.ux[kp.nxt][p], U1 = Mp * f.uy[kp.nxt][p], U2 = Mp;
+    const double Mp = at(f.Mp, p);
+    U0 = Mp * at(f.ux[kp.nxt], p), U1 = Mp * at(f.uy[kp.nxt], p), U2 = Mp;
   } else {
-    U0 = f.UD1[0][p], U1 = f.UD1[1][p], U2 = f.UD1[2][p];
+    U0 = at(f.UD1[0], p), U1 = at(f.UD1[1], p), U2 = at(f.UD1[2], p);
   }
   const double R2 = U2 + minus - plus;
   const double R0 = U0 + minus * uvm[0] - plus * uvp[0];
   const double R1 = U1 + minus * uvm[1] - plus * uvp[1];
   if (kp.sweep == 1) {
-    f.UD1[0][p] = R0, f.UD1[1][p] = R1, f.UD1[2][p] = R2;
+    at(f.UD1[0], p) = R0, at(f.UD1[1], p) = R1, at(f.UD1[2], p) = R2;
   } else {
     // remapVariables, nodes (PhaseRemapFinal.cc:207-234)
-    f.ux[kp.nxt][p] = R0 / R2;
-    f.uy[kp.nxt][p] = R1 / R2;
+    at(f.ux[kp.nxt], p) = R0 / R2;
+    at(f.uy[kp.nxt], p) = R1 / R2;
   }
 }
 
@@ -259,15 +261,17 @@ __device__ __forceinline__ void dualNodeFinish(const KPT<R>& kp, long a, long t,
     }
     auto gidx = [&](long b, long tt) { return XD ? tt * (nx + 1) + b : b * nx + tt; };
     auto cidx = [&](long b, long tt) { return XD ? tt * nx + b : b * nx + tt; };
+    // the along-sweep neighbour's face / cell: a uniform step, folded into the base
+    const long dA = XD ? 1 : nx;
     for (int k = 0; k < m; ++k) {
       double pk = 0, mk = 0;
       if (o.methode == 0 || o.methode == 1) {
         for (int s2 = 0; s2 < nf; ++s2) {
           const long tt = tf[s2];
-          const double Gf = f.G[k][gidx(a, tt)];
-          double tp = Gf + f.G[k][gidx(a + 1, tt)];
-          double tm = Gf + f.G[k][gidx(a - 1, tt)];
-          if (o.methode == 1) tp = f.rL[cidx(a, tt)] * tp, tm = f.rL[cidx(a - 1, tt)] * tm;
+          const double Gf = at(f.G[k], gidx(a, tt));
+          double tp = Gf + at(f.G[k] + dA, gidx(a, tt));
+          double tm = Gf + at(f.G[k] - dA, gidx(a, tt));
+          if (o.methode == 1) tp = at(f.rL, cidx(a, tt)) * tp, tm = at(f.rL - dA, cidx(a, tt)) * tm;
           pk += tp;
           mk += tm;
         }
@@ -277,8 +281,8 @@ __device__ __forceinline__ void dualNodeFinish(const KPT<R>& kp, long a, long t,
         int nbcell = 0;
         for (int s2 = 0; s2 < nf; ++s2) {
           const long tt = tf[s2];
-          pk += f.DF[k][cidx(a, tt)];
-          mk += f.DF[k][cidx(a - 1, tt)];
+          pk += at(f.DF[k], cidx(a, tt));
+          mk += at(f.DF[k] - dA, cidx(a, tt));
           nbcell++;
         }
         if (nbcell != 0) pk /= nbcell, mk /= nbcell;
@@ -309,20 +313,20 @@ __device__ __forceinline__ void dualNodeFinish(const KPT<R>& kp, long a, long t,
   upwind(false, plus, gP, uvp);
   double U0, U1, U2;
   if (kp.sweep == 1) {
-    const double Mp = f.Mp[p];
-    U0 = Mp * f.ux[kp.nxt][p], U1 = Mp * f.uy[kp.nxt][p], U2 = Mp;
+    const double Mp = at(f.Mp, p);
+    U0 = Mp * at(f.ux[kp.nxt], p), U1 = Mp * at(f.uy[kp.nxt], p), U2 = Mp;
   } else {
-    U0 = f.UD1[0][p], U1 = f.UD1[1][p], U2 = f.UD1[2][p];
+    U0 = at(f.UD1[0], p), U1 = at(f.UD1[1], p), U2 = at(f.UD1[2], p);
   }
   const double R2 = U2 + minus - plus;
   const double R0 = U0 + minus * uvm[0] - plus * uvp[0];
   const double R1 = U1 + minus * uvm[1] - plus * uvp[1];
   if (kp.sweep == 1) {
-    f.UD1[0][p] = R0, f.UD1[1][p] = R1, f.UD1[2][p] = R2;
+    at(f.UD1[0], p) = R0, at(f.UD1[1], p) = R1, at(f.UD1[2], p) = R2;
   } else {
     // remapVariables, nodes (PhaseRemapFinal.cc:207-234)
-    f.ux[kp.nxt][p] = R0 / R2;
-    f.uy[kp.nxt][p] = R1 / R2;
+    at(f.ux[kp.nxt], p) = R0 / R2;
+    at(f.uy[kp.nxt], p) = R1 / R2;
   }
 }
 
@@ -366,17 +370,17 @@ __device__ __forceinline__ void dualWaveBody(const KPT<R>& kp, unsigned b) {
   double xl0 = 0.0;
   if (in) {
     if (kp.sweep == 1) {
-      const double ux1 = (double)f.ux[kp.nxt][p], uy1 = (double)f.uy[kp.nxt][p];
+      const double ux1 = (double)at(f.ux[kp.nxt], p), uy1 = (double)at(f.uy[kp.nxt], p);
       dpx0 = T(ux1), dpy0 = T(uy1);
       const D2 e = eulerNode(o, j, i);
       xl0 = XD ? e.x + ux1 * dt1 : e.y + uy1 * dt1;
       // Lagrange coordinate along the second sweep's axis (the second sweep
       // overwrites the velocity in place)
-      if (output) f.XL2[p] = XD ? e.y + uy1 * dt1 : e.x + ux1 * dt1;
+      if (output) at(f.XL2, p) = XD ? e.y + uy1 * dt1 : e.x + ux1 * dt1;
     } else {
-      const double M1 = f.UD1[2][p];
-      dpx0 = T(f.UD1[0][p] / M1), dpy0 = T(f.UD1[1][p] / M1);
-      xl0 = f.XL2[p];
+      const double M1 = at(f.UD1[2], p);
+      dpx0 = T(at(f.UD1[0], p) / M1), dpy0 = T(at(f.UD1[1], p) / M1);
+      xl0 = at(f.XL2, p);
     }
   }
   // nodes a-2 .. a+2 (slots 1..5 of the reference stencil a-3 .. a+3)
@@ -480,15 +484,15 @@ __device__ __forceinline__ void dualWalkBody(const KPT<R>& kp, long item) {
     if (rin && inT) {
       const long p = r * (nx + 1) + t;
       if (kp.sweep == 1) {
-        const double ux1 = (double)f.ux[kp.nxt][p], uy1 = (double)f.uy[kp.nxt][p];
+        const double ux1 = (double)at(f.ux[kp.nxt], p), uy1 = (double)at(f.uy[kp.nxt], p);
         dx = T(ux1), dy = T(uy1);
         const D2 e = eulerNode(o, r, t);
         xl = e.y + uy1 * dt1;
-        if (r >= a0 && r <= a1) f.XL2[p] = e.x + ux1 * dt1;  // the second sweep's axis
+        if (r >= a0 && r <= a1) at(f.XL2, p) = e.x + ux1 * dt1;  // the second sweep's axis
       } else {
-        const double M1 = f.UD1[2][p];
-        dx = T(f.UD1[0][p] / M1), dy = T(f.UD1[1][p] / M1);
-        xl = f.XL2[p];
+        const double M1 = at(f.UD1[2], p);
+        dx = T(at(f.UD1[0], p) / M1), dy = T(at(f.UD1[1], p) / M1);
+        xl = at(f.XL2, p);
       }
     }
     wx[6] = dx, wy[6] = dy, wl[6] = xl, wa[6] = rin;
@@ -586,17 +590,17 @@ __device__ __forceinline__ void dualTileBody(const KPT<R>& kp, unsigned b) {
     if (inRow(L) && inT) {
       const long p = a * (nx + 1) + t;
       if (kp.sweep == 1) {
-        const double ux1 = (double)f.ux[kp.nxt][p], uy1 = (double)f.uy[kp.nxt][p];
+        const double ux1 = (double)at(f.ux[kp.nxt], p), uy1 = (double)at(f.uy[kp.nxt], p);
         dx = T(ux1), dy = T(uy1);
         const D2 e = eulerNode(o, a, t);
         xl = e.y + uy1 * dt1;
         // Lagrange coordinate along the second sweep's axis (the second sweep
         // overwrites the velocity in place)
-        if (L >= 3 && L < 3 + kDTOut) f.XL2[p] = e.x + ux1 * dt1;
+        if (L >= 3 && L < 3 + kDTOut) at(f.XL2, p) = e.x + ux1 * dt1;
       } else {
-        const double M1 = f.UD1[2][p];
-        dx = T(f.UD1[0][p] / M1), dy = T(f.UD1[1][p] / M1);
-        xl = f.XL2[p];
+        const double M1 = at(f.UD1[2], p);
+        dx = T(at(f.UD1[0], p) / M1), dy = T(at(f.UD1[1], p) / M1);
+        xl = at(f.XL2, p);
       }
     }
     sdx[L][col] = dx, sdy[L][col] = dy, sxl[L][col] = xl;
@@ -701,7 +705,7 @@ __global__ __launch_bounds__(256) void k_remap_final(KPT<R> kp) {
   if (c >= kp.ncell) return;
   double U[NC];
 #pragma unroll
-  for (int q = 0; q < NC; ++q) U[q] = kp.f.U2[q][c];
+  for (int q = 0; q < NC; ++q) U[q] = at(kp.f.U2[q], c);
   finalCell<R, M>(kp, c, U);
 }
 
@@ -717,8 +721,8 @@ __global__ void k_node_mass(KPT<R> kp) {
   double s = 0.0;
   int n = 0;
   for (int q = 0; q < 4; ++q)
-    if (hc[q]) s = s + kp.f.M[cc[q]], ++n;
-  kp.f.Mp[p] = s / (double)n;
+    if (hc[q]) s = s + at(kp.f.M, cc[q]), ++n;
+  at(kp.f.Mp, p) = s / (double)n;
 }
 
 // Conservation sums with Neumaier compensation: out[0] mass, out[1]
@@ -734,14 +738,14 @@ __global__ __launch_bounds__(256) void k_env_expand(KPT<R> kp) {
   const auto& f = kp.f;
   const int s = kp.cur;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < kp.ncell; c += (long)gridDim.x * blockDim.x) {
-    if (f.envx[c]) continue;
-    f.f_k[0][c] = 1.0;
-    f.mu_k[0][c] = R(1.0);
-    f.M_k[0][c] = f.M[c];
-    f.rho_k[s][0][c] = f.rho[s][c];
-    f.e_k[s][0][c] = f.e[s][c];
-    f.p_k[s][0][c] = f.p[s][c];
-    f.c_k[s][0][c] = f.c[s][c];
+    if (at(f.envx, c)) continue;
+    at(f.f_k[0], c) = 1.0;
+    at(f.mu_k[0], c) = R(1.0);
+    at(f.M_k[0], c) = at(f.M, c);
+    at(f.rho_k[s][0], c) = at(f.rho[s], c);
+    at(f.e_k[s][0], c) = at(f.e[s], c);
+    at(f.p_k[s][0], c) = at(f.p[s], c);
+    at(f.c_k[s][0], c) = at(f.c[s], c);
   }
 }
 
@@ -759,12 +763,12 @@ __global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
     s[q] = t;
   };
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < kp.ncell; c += (long)gridDim.x * blockDim.x) {
-    const double M = f.M[c];
+    const double M = at(f.M, c);
     add(0, M);
-    add(1, M * (double)f.e[kp.cur][c]);
+    add(1, M * (double)at(f.e[kp.cur], c));
     // (compact1: the material mass is the cell mass unless stored)
-    const bool derived = kp.o.compact1 && !f.envx[c];
-    for (int k = 0; k < m; ++k) add(2 + k, derived ? M : f.M_k[k][c]);
+    const bool derived = kp.o.compact1 && !at(f.envx, c);
+    for (int k = 0; k < m; ++k) add(2 + k, derived ? M : at(f.M_k[k], c));
   }
   __shared__ double sh[256];
   for (int q = 0; q < nq; ++q) {
