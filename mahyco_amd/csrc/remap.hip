@@ -547,9 +547,12 @@ dim3 dualWalkGrid(long nx, long ny) {
 }
 
 // Nodal remap along y on an LDS tile: a 256-thread block covers 16 node
-// columns and 64 node rows (3 halo rows above and below, 58 rows updated),
-// each thread 4 rows of one column, so that every load and store instruction
+// columns and 48 node rows (3 halo rows above and below, 42 rows updated),
+// each thread 3 rows of one column, so that every load and store instruction
 // of a wave touches 4 whole row segments of 16 nodes (128 bytes in FP64).
+// 48 rows keep the tile at 30 KB of LDS and 86 VGPRs: 5 blocks per CU (64
+// rows, 40 KB: 4 blocks, and 20 % slower on a 4096^2 mesh; 32 rows: more
+// halo, 5 % slower than 48).
 // The block loads DualPhi and the Lagrange coordinate of the tile into LDS,
 // forms each row's limited gradient from its neighbours there, and updates
 // the inner rows; every value comes from the same operands as in the
@@ -557,9 +560,9 @@ dim3 dualWalkGrid(long nx, long ny) {
 namespace {
 constexpr int kDTThreads = 256;                 // threads per tile (512 and 1024: no faster)
 constexpr int kDTCols = 16;                     // node columns per tile
-constexpr int kDTRows = 64;                     // node rows per tile
+constexpr int kDTRows = 48;                     // node rows per tile
 constexpr int kDTRowsPer = kDTRows * kDTCols / kDTThreads;  // rows per thread
-constexpr int kDTOut = kDTRows - 6;             // rows updated (58)
+constexpr int kDTOut = kDTRows - 6;             // rows updated (42)
 }  // namespace
 
 template <class R>
