@@ -801,8 +801,6 @@ static int setupEucclhyd(mhy_ctx* x, const mhy_config& c, bool batchMember) {
       return fail(x, MHY_ERR_ARG,
                   "equation of state " + std::to_string(c.eos[k]) +
                       " is not programmed for the Eucclhyd scheme (PhaseLagrange.cc:111-135)");
-  if (c.nx < 1 || c.ny < 1 || (long)(c.nx + 1) * (c.ny + 1) >= (1L << 29))
-    return fail(x, MHY_ERR_ARG, "mesh size out of range (at most 2^29 nodes)");
   x->euc = true;
   x->mixed = false;
   x->sweepWave = true;
@@ -940,6 +938,11 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
     *out = x;
     return code;
   };
+  // the kernels index cells and nodes with 32-bit arithmetic and address
+  // every array with 32-bit byte offsets (FP64 node arrays below 4 GiB);
+  // checked before any host or device allocation
+  if (c.nx < 1 || c.ny < 1 || (long)(c.nx + 1) * (c.ny + 1) >= (1L << 29))
+    return bail(fail(x, MHY_ERR_ARG, "mesh size out of range (at most 2^29 nodes)"));
   if (c.schema == MHY_SCHEMA_EUCCLHYD) {
     const int st = setupEucclhyd(x, c, batchMember);
     *out = x;
@@ -1023,10 +1026,6 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
   if (c.avec_projection && !outerNormalsAxisExact(c, err)) return bail(fail(x, MHY_ERR_ARG, err));
 
   const HostInit& h = x->init;
-  // the kernels index cells and nodes with 32-bit arithmetic and address
-  // every array with 32-bit byte offsets (FP64 node arrays below 4 GiB)
-  if (c.nx < 1 || c.ny < 1 || (long)(c.nx + 1) * (c.ny + 1) >= (1L << 29))
-    return bail(fail(x, MHY_ERR_ARG, "mesh size out of range (at most 2^29 nodes)"));
   x->ncell = (long)c.nx * c.ny;
   x->nnode = (long)(c.nx + 1) * (c.ny + 1);
   const long C = x->ncell, N = x->nnode;

@@ -97,3 +97,17 @@ def test_batch_rejects_mixed_material_counts(lib):
     b = configs.triple_layout(30, 10)
     with pytest.raises(mahyco_amd.MahycoError, match="material count"):
         mahyco_amd.Batch([a, b])
+
+
+@pytest.mark.parametrize("schema", ["vnr", "euc"])
+def test_oversized_mesh_is_rejected_before_any_allocation(lib, schema):
+    """The kernels address every array with 32-bit byte offsets, so a context
+    takes at most 2^29 nodes; the check runs before any host or device
+    allocation (no GPU needed)."""
+    from mahyco_amd import configs
+    cfg = configs.point_explosion(64)
+    if schema == "euc":
+        cfg.schema = abi.SCHEMA_EUCCLHYD
+    cfg.nx = cfg.ny = 23170  # 23171^2 nodes > 2^29
+    with pytest.raises(mahyco_amd.MahycoError, match="mesh size out of range"):
+        mahyco_amd.Context(cfg)
