@@ -302,12 +302,14 @@ long long mhy_batch_cells(const mhy_batch* batch);
 int mhy_kernel_count(const mhy_ctx* ctx);
 /* The launch-shape tuner: on meshes of at least 2^20 cells the first steps
  * of mhy_step / mhy_run / mhy_simulate time the interchangeable y-direction
- * kernels (nodal remap on 16x64-node LDS tiles, 4x16-node wave strips or
+ * kernels (nodal remap on 16x48-node LDS tiles, 4x16-node wave strips or
  * 64-column walking bands; cell sweep on 4x16-cell wave strips or 64-column
- * walking bands; results bitwise the same) and keep the fastest before any
- * graph is captured.  Returns the kept variants ("dual_y=tile sweep_y=wave"),
- * or "" while tuning or when the tuner is off (MAHYCO_AUTOTUNE=0).  No
- * counterpart in the reference. */
+ * walking bands; the Lagrange update of y-first steps fused into the
+ * first sweep or run as its own kernel; results bitwise the same) and keep
+ * the fastest before any graph is captured.  Returns the kept variants
+ * ("dual_y=tile sweep_y=wave lagrange_y=fused") and a Lagrange variant forced
+ * through the environment (MAHYCO_LAG_Y), or "" while tuning or when the
+ * tuner is off (MAHYCO_AUTOTUNE=0).  No counterpart in the reference. */
 const char* mhy_tuned_kernels(const mhy_ctx* ctx);
 const char* mhy_kernel_name(const mhy_ctx* ctx, int k);
 int mhy_kernel_bytes(const mhy_ctx* ctx, int k, double* bytes_per_launch);

@@ -97,6 +97,7 @@ struct mhy_ctx {
   bool fuseLag = true;
   // ... also on steps whose first sweep runs along y (the tuner may turn it
   // off there: the y strips recompute a third of their cells as halo)
+  // (MAHYCO_LAG_Y=fused / separate forces either)
   bool fuseLagY = true;
   // with the fused first sweep: that sweep stores the Lagrange volume and
   // centroid for the second (environment MAHYCO_SWEEP_GEOM=load), or the
@@ -113,7 +114,7 @@ struct mhy_ctx {
   // faster (fewer waves leave the chip underfilled); MAHYCO_DUAL_Y=walk /
   // wave / node forces a kernel
   bool dualWalkY = false;
-  // nodal remap along y on LDS tiles of 16 node columns x 64 rows
+  // nodal remap along y on LDS tiles of 16 node columns x 48 rows
   // (MAHYCO_DUAL_Y=tile)
   bool dualTileY = false;
   // cell sweeps along y: the column-walking kernel or the wave-strip
@@ -167,6 +168,7 @@ struct mhy_ctx {
     double ms[3][kTuneRounds][3] = {};  // two-step time per site, round and candidate
     hipEvent_t ev[2] = {nullptr, nullptr};  // around the current step
     std::string choice;  // the kept variants, for reports
+    std::string forcedLag;  // the y-first Lagrange kernel forced through MAHYCO_LAG_Y
   } tune;
   // output
   OutputWriter* writer = nullptr;
@@ -330,6 +332,7 @@ void tuneStepDone(mhy_ctx* x) {
     if (t.ncand[0]) add("dual_y", kDualNames[t.cand[0][t.best[0]]]);
     if (t.ncand[1]) add("sweep_y", kSweepNames[t.cand[1][t.best[1]]]);
     if (t.ncand[2]) add("lagrange_y", kLagNames[t.cand[2][t.best[2]]]);
+    else if (!t.forcedLag.empty()) add("lagrange_y", t.forcedLag.c_str());
   }
 }
 
@@ -985,6 +988,11 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
                (!x->walkY || x->geomStored);
   if (const char* fl = std::getenv("MAHYCO_FUSE_LAGRANGE"))
     if (std::string(fl) == "0") x->fuseLag = false;
+  if (const char* ly = std::getenv("MAHYCO_LAG_Y")) {
+    x->fuseLagY = (std::string(ly) != "separate");
+    x->tune.forcedLag = kLagNames[x->fuseLagY ? TL_FUSED : TL_SEPARATE];
+    x->tune.choice = "lagrange_y=" + x->tune.forcedLag;
+  }
   if (const char* sn = std::getenv("MAHYCO_SPLIT_NODE")) x->splitNode = (std::string(sn) != "0");
   // the tuner (MAHYCO_AUTOTUNE=0 disables it; a variant forced through the
   // environment is not tuned): meshes of at least 2^20 cells, where the
@@ -1009,7 +1017,7 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
       t.cand[1][1] = x->walkY ? TS_WAVE : TS_WALK;
       t.ncand[1] = 2;
     }
-    if (tuneOn && x->fuseLag) {
+    if (tuneOn && x->fuseLag && !std::getenv("MAHYCO_LAG_Y")) {
       t.cand[2][0] = TL_FUSED, t.cand[2][1] = TL_SEPARATE;
       t.ncand[2] = 2;
     }

@@ -79,7 +79,7 @@ dim3 dualWaveGrid(bool xd, long nx, long ny, bool mixed);
 // nodal remap along y with one lane per node column walking down a band
 template <class R>
 __global__ void k_dual_walk(KPT<R> kp);
-// nodal remap along y on LDS tiles of 16 node columns x 64 rows
+// nodal remap along y on LDS tiles of 16 node columns x 48 rows
 template <class R>
 __global__ void k_dual_tile(KPT<R> kp);
 dim3 dualTileGrid(long nx, long ny);

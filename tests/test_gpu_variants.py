@@ -10,7 +10,8 @@ tile), the Lagrange volume and centroid formed again in the second sweep or
 stored by the first (MAHYCO_SWEEP_GEOM=recompute, load), one node
 kernel for interior and boundary nodes (MAHYCO_SPLIT_NODE=0), and a
 one-material context storing its material fields in every cell
-(MAHYCO_COMPACT=0), in both storage forms.  The choice is read when a
+(MAHYCO_COMPACT=0), the separate Lagrange cell kernel on the steps whose first
+sweep runs along y only (MAHYCO_LAG_Y=separate), in both storage forms.  The choice is read when a
 context is created."""
 import os
 from contextlib import contextmanager
@@ -31,7 +32,8 @@ VARIANTS = {"cell_sweeps": ("MAHYCO_SWEEP_KERNEL", "cell"), "separate_final": ("
             "wave_dual_y": ("MAHYCO_DUAL_Y", "wave"), "tile_dual_y": ("MAHYCO_DUAL_Y", "tile"),
             "recomputed_geometry": ("MAHYCO_SWEEP_GEOM", "recompute"),
             "stored_geometry": ("MAHYCO_SWEEP_GEOM", "load"), "one_node_kernel": ("MAHYCO_SPLIT_NODE", "0"),
-            "stored_material_fields": ("MAHYCO_COMPACT", "0")}
+            "stored_material_fields": ("MAHYCO_COMPACT", "0"),
+            "separate_lagrange_y": ("MAHYCO_LAG_Y", "separate")}
 NAMES = ["C2_pp_superbee", "C2_pp_PB", "C2_pp_mixte_arithmeticG", "C3_point_explosion", "C4_triple_layout",
          "riemann_order1", "riemann_superBeeG_main", "unit_test_boundary_flux", "riemann_order3"]
 
