@@ -114,9 +114,10 @@ struct mhy_ctx {
   // faster (fewer waves leave the chip underfilled); MAHYCO_DUAL_Y=walk /
   // wave / node forces a kernel
   bool dualWalkY = false;
-  // nodal remap along y on LDS tiles of 16 node columns x 48 rows
-  // (MAHYCO_DUAL_Y=tile)
+  // nodal remap along y on LDS tiles of 16 node columns x dualTileRows
+  // rows: 48 (MAHYCO_DUAL_Y=tile), 32 or 64 (tile32, tile64)
   bool dualTileY = false;
+  int dualTileRows = 48;
   // cell sweeps along y: the column-walking kernel or the wave-strip
   // kernel.  The walker keeps more state in registers (fewer waves per SIMD)
   // but has less halo work; it is chosen for the mixed form with one
@@ -162,10 +163,11 @@ struct mhy_ctx {
   struct Tuner {
     bool on = false;
     int step = 0, steps = 0;
+    static constexpr int kMaxCand = 5;
     int ncand[3] = {0, 0, 0};  // site 0: nodal remap along y, 1: cell sweep along y, 2: y-first Lagrange
-    int cand[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    int cand[3][kMaxCand] = {};
     int best[3] = {0, 0, 0};  // index into cand of the current choice
-    double ms[3][kTuneRounds][3] = {};  // two-step time per site, round and candidate
+    double ms[3][kTuneRounds][kMaxCand] = {};  // two-step time per site, round and candidate
     hipEvent_t ev[2] = {nullptr, nullptr};  // around the current step
     std::string choice;  // the kept variants, for reports
     std::string forcedLag;  // the y-first Lagrange kernel forced through MAHYCO_LAG_Y
@@ -245,15 +247,16 @@ KP makeKP(const mhy_ctx* x) {
 }
 
 // candidate ids of the tuner's sites
-enum { TD_TILE = 0, TD_WAVE = 1, TD_WALK = 2 };  // nodal remap along y
+enum { TD_TILE = 0, TD_WAVE = 1, TD_WALK = 2, TD_TILE32 = 3, TD_TILE64 = 4 };  // nodal remap along y
 enum { TS_WAVE = 0, TS_WALK = 1 };               // cell sweep along y
 enum { TL_FUSED = 0, TL_SEPARATE = 1 };          // Lagrange cell update of y-first steps
-const char* kDualNames[3] = {"tile", "wave", "walk"};
+const char* kDualNames[5] = {"tile", "wave", "walk", "tile32", "tile64"};
 const char* kSweepNames[2] = {"wave", "walk"};
 const char* kLagNames[2] = {"fused", "separate"};
 
 void setDualY(mhy_ctx* x, int c) {
-  x->dualTileY = (c == TD_TILE);
+  x->dualTileY = (c == TD_TILE || c == TD_TILE32 || c == TD_TILE64);
+  x->dualTileRows = (c == TD_TILE32) ? 32 : (c == TD_TILE64) ? 64 : 48;
   x->dualWalkY = (c == TD_WALK);
   x->dualWaveY = true;
 }
@@ -613,7 +616,11 @@ int phaseSweepStage(mhy_ctx* x, int sweep, int stage) {
           [&] { launchR(x, k_dual<double, true>, k_dual<float, true>, gridFor(x->nnode), dim3(kBlock), kp); });
   } else if (x->dualTileY) {
     timed(x, K_DUAL_Y, [&] {
-      launchR(x, k_dual_tile<double>, k_dual_tile<float>, dualTileGrid(x->o.nx, x->o.ny), dim3(dualTileBlock()), kp);
+      const int rows = x->dualTileRows;
+      const dim3 grid = dualTileGrid(x->o.nx, x->o.ny, rows), block(dualTileBlock());
+      if (rows == 32) launchR(x, k_dual_tile<double, 32>, k_dual_tile<float, 32>, grid, block, kp);
+      else if (rows == 64) launchR(x, k_dual_tile<double, 64>, k_dual_tile<float, 64>, grid, block, kp);
+      else launchR(x, k_dual_tile<double, 48>, k_dual_tile<float, 48>, grid, block, kp);
     });
   } else if (x->dualWalkY) {
     timed(x, K_DUAL_Y, [&] {
@@ -969,7 +976,9 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
   if (const char* dy = std::getenv("MAHYCO_DUAL_Y")) {
     x->dualWaveY = (std::string(dy) != "node");
     x->dualWalkY = (std::string(dy) == "walk");
-    x->dualTileY = (std::string(dy) == "tile");
+    const std::string v(dy);
+    x->dualTileY = (v == "tile" || v == "tile32" || v == "tile64");
+    x->dualTileRows = (v == "tile32") ? 32 : (v == "tile64") ? 64 : 48;
   }
   if (const char* wy = std::getenv("MAHYCO_SWEEP_Y")) {
     if (std::string(wy) == "walk") x->walkY = true;
@@ -1006,7 +1015,8 @@ static int createContext(const mhy_config* cfg, mhy_ctx** out, bool batchMember)
     if (tuneOn && !std::getenv("MAHYCO_DUAL_Y")) {
       t.cand[0][0] = x->dualTileY ? TD_TILE : (x->dualWalkY ? TD_WALK : TD_WAVE);
       int n = 1;
-      for (int c2 : {TD_TILE, TD_WAVE, TD_WALK})
+      // (the tile heights: 32 and 64 rows besides the default 48)
+      for (int c2 : {TD_TILE, TD_WAVE, TD_WALK, TD_TILE32, TD_TILE64})
         if (c2 != t.cand[0][0]) t.cand[0][n++] = c2;
       t.ncand[0] = n;
     }

@@ -79,10 +79,10 @@ dim3 dualWaveGrid(bool xd, long nx, long ny, bool mixed);
 // nodal remap along y with one lane per node column walking down a band
 template <class R>
 __global__ void k_dual_walk(KPT<R> kp);
-// nodal remap along y on LDS tiles of 16 node columns x 48 rows
-template <class R>
+// nodal remap along y on LDS tiles of 16 node columns x ROWS rows (32, 48, 64)
+template <class R, int ROWS>
 __global__ void k_dual_tile(KPT<R> kp);
-dim3 dualTileGrid(long nx, long ny);
+dim3 dualTileGrid(long nx, long ny, int rows);
 unsigned dualTileBlock();
 dim3 dualWalkGrid(long nx, long ny);
 unsigned dualWaveBlock(bool xd, bool mixed);

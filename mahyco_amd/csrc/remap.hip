@@ -547,12 +547,12 @@ dim3 dualWalkGrid(long nx, long ny) {
 }
 
 // Nodal remap along y on an LDS tile: a 256-thread block covers 16 node
-// columns and 48 node rows (3 halo rows above and below, 42 rows updated),
-// each thread 3 rows of one column, so that every load and store instruction
-// of a wave touches 4 whole row segments of 16 nodes (128 bytes in FP64).
-// 48 rows keep the tile at 30 KB of LDS and 86 VGPRs: 5 blocks per CU (64
-// rows, 40 KB: 4 blocks, and 20 % slower on a 4096^2 mesh; 32 rows: more
-// halo, 5 % slower than 48).
+// columns and ROWS node rows (3 halo rows above and below, ROWS - 6 rows
+// updated), each thread ROWS / 16 rows of one column, so that every load and
+// store instruction of a wave touches 4 whole row segments of 16 nodes (128
+// bytes in FP64).  The tuner picks the tile height: 48 rows keep the tile at
+// 30 KB of LDS and 86 VGPRs, 5 blocks per CU (on a 4096^2 mesh 64 rows, 40
+// KB and 4 blocks, are 20 % slower, 32 rows, with more halo, 5 % slower).
 // The block loads DualPhi and the Lagrange coordinate of the tile into LDS,
 // forms each row's limited gradient from its neighbours there, and updates
 // the inner rows; every value comes from the same operands as in the
@@ -560,13 +560,14 @@ dim3 dualWalkGrid(long nx, long ny) {
 namespace {
 constexpr int kDTThreads = 256;                 // threads per tile (512 and 1024: no faster)
 constexpr int kDTCols = 16;                     // node columns per tile
-constexpr int kDTRows = 48;                     // node rows per tile
-constexpr int kDTRowsPer = kDTRows * kDTCols / kDTThreads;  // rows per thread
-constexpr int kDTOut = kDTRows - 6;             // rows updated (42)
 }  // namespace
 
-template <class R>
+template <class R, int ROWS>
 __device__ __forceinline__ void dualTileBody(const KPT<R>& kp, unsigned b) {
+  static_assert(ROWS % (kDTThreads / kDTCols) == 0 && ROWS > 6, "whole passes of 16 rows");
+  constexpr int kDTRows = ROWS;
+  constexpr int kDTRowsPer = kDTRows * kDTCols / kDTThreads;  // rows per thread
+  constexpr int kDTOut = kDTRows - 6;                         // rows updated
   using T = typename std::conditional<kMixed<R>, float, double>::type;
   const Opts& o = kp.o;
   const auto& f = kp.f;
@@ -609,7 +610,7 @@ __device__ __forceinline__ void dualTileBody(const KPT<R>& kp, unsigned b) {
     sdx[L][col] = dx, sdy[L][col] = dy, sxl[L][col] = xl;
   }
   __syncthreads();
-  // limited gradients of the rows the updates need (tile rows 2 .. 61)
+  // limited gradients of the rows the updates need (tile rows 2 .. ROWS - 3)
 #pragma unroll
   for (int k = 0; k < kDTRowsPer; ++k) {
     const int L = tileRow(k);
@@ -656,17 +657,19 @@ __device__ __forceinline__ void dualTileBody(const KPT<R>& kp, unsigned b) {
   }
 }
 
-template <class R>
+template <class R, int ROWS>
 __global__ __launch_bounds__(kDTThreads) void k_dual_tile(KPT<R> kp) {
   // neighbouring column groups share the cache lines of the face fluxes
   // left of their first column: one XCD (one L2) takes a contiguous range
-  dualTileBody<R>(kp, xcdBlock());
+  dualTileBody<R, ROWS>(kp, xcdBlock());
 }
 
 unsigned dualTileBlock() { return kDTThreads; }
 
-dim3 dualTileGrid(long nx, long ny) {
-  const long gx = (nx + 1 + kDTCols - 1) / kDTCols, gy = (ny + 1 + kDTOut - 1) / kDTOut;
+// the grid of tiles of `rows` node rows
+dim3 dualTileGrid(long nx, long ny, int rows) {
+  const long out = rows - 6;
+  const long gx = (nx + 1 + kDTCols - 1) / kDTCols, gy = (ny + 1 + out - 1) / out;
   return dim3((unsigned)(gx * gy));
 }
 
@@ -815,7 +818,9 @@ __global__ __launch_bounds__(256) void k_conservation(KPT<R> kp, double* out) {
   template __global__ void k_node_mass<R>(KPT<R>);                     \
   template __global__ void k_conservation<R>(KPT<R>, double*);         \
   template __global__ void k_env_expand<R>(KPT<R>);                     \
-  template __global__ void k_dual_tile<R>(KPT<R>);
+  template __global__ void k_dual_tile<R, 32>(KPT<R>);              \
+  template __global__ void k_dual_tile<R, 48>(KPT<R>);              \
+  template __global__ void k_dual_tile<R, 64>(KPT<R>);
 MHY_REMAP_INST(double)
 MHY_REMAP_INST(float)
 #undef MHY_REMAP_INST

@@ -1,6 +1,6 @@
 """The launch-shape tuner: on a mesh of at least 2^20 cells the first steps
 time the interchangeable y-direction kernel variants (nodal remap on LDS
-tiles, wave strips or walking bands; cell sweep on wave strips or walking
+tiles of 48, 32 or 64 node rows, wave strips or walking bands; cell sweep on wave strips or walking
 bands; the Lagrange update of y-first steps fused or separate) and keep the
 fastest.  Results must be bitwise those of the same problem with the tuner
 off, through the tuning steps, the switch to the kept variants and the
@@ -50,7 +50,7 @@ def test_tuned_run_equals_untuned(name, precision):
     try:
         assert tuned.tuned_kernels() == ""  # still tuning
         done = 0
-        for chunk in (3, 25, 6):  # tuning steps (at most 28), then graph replays
+        for chunk in (3, 33, 6):  # tuning steps (at most 36), then graph replays
             tuned.run(chunk)
             plain.run(chunk)
             done += chunk

@@ -6,7 +6,7 @@ fusion into the first sweep (MAHYCO_FUSE_LAGRANGE=0), the per-node dual
 remap along y (MAHYCO_DUAL_Y=node,
 FP64 form: in the mixed form it keeps FP64 velocity arithmetic) and the
 column-walking and LDS-tile dual remaps along y (MAHYCO_DUAL_Y=walk,
-tile), the Lagrange volume and centroid formed again in the second sweep or
+tile, and the 32- and 64-row tiles tile32, tile64), the Lagrange volume and centroid formed again in the second sweep or
 stored by the first (MAHYCO_SWEEP_GEOM=recompute, load), one node
 kernel for interior and boundary nodes (MAHYCO_SPLIT_NODE=0), and a
 one-material context storing its material fields in every cell
@@ -30,6 +30,7 @@ VARIANTS = {"cell_sweeps": ("MAHYCO_SWEEP_KERNEL", "cell"), "separate_final": ("
             "separate_lagrange": ("MAHYCO_FUSE_LAGRANGE", "0"),
             "node_dual_y": ("MAHYCO_DUAL_Y", "node"), "walk_dual_y": ("MAHYCO_DUAL_Y", "walk"),
             "wave_dual_y": ("MAHYCO_DUAL_Y", "wave"), "tile_dual_y": ("MAHYCO_DUAL_Y", "tile"),
+            "tile32_dual_y": ("MAHYCO_DUAL_Y", "tile32"), "tile64_dual_y": ("MAHYCO_DUAL_Y", "tile64"),
             "recomputed_geometry": ("MAHYCO_SWEEP_GEOM", "recompute"),
             "stored_geometry": ("MAHYCO_SWEEP_GEOM", "load"), "one_node_kernel": ("MAHYCO_SPLIT_NODE", "0"),
             "stored_material_fields": ("MAHYCO_COMPACT", "0"),
