@@ -26,6 +26,10 @@ HOST_SOURCES = ["host/donnees.cc", "host/json_input.cc", "host/init.cc", "host/i
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
+# per-source device flags: the column walker loops over rows, and
+# machine-level hoisting of constants out of that loop would hold registers
+# through every row
+SOURCE_FLAGS = {"sweep_walk.hip": ["-mllvm", "-disable-machine-licm"]}
 
 
 def hipcc():
@@ -51,7 +55,7 @@ def build(verbose=False):
     for s in HIP_SOURCES:
         src = os.path.join(SRC, s)
         obj = os.path.join(OBJ, s.replace("/", "_") + ".o")
-        jobs.append((src, obj, [cc, "-c", src, "-o", obj] + COMMON + DEVICE))
+        jobs.append((src, obj, [cc, "-c", src, "-o", obj] + COMMON + DEVICE + SOURCE_FLAGS.get(s, [])))
     for s in HOST_SOURCES:
         src = os.path.join(SRC, s)
         obj = os.path.join(OBJ, s.replace("/", "_") + ".o")

@@ -58,9 +58,8 @@ __device__ __forceinline__ void sweepWalkYBody(const KPT<R>& kp, unsigned b) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long ngx = (nx + 63) / 64;
   const long g = (long)b * 4 + wave;
-  const long i = (g % ngx) * 64 + lane;
+  const long i0 = (g % ngx) * 64 + lane;
   const long a0 = (g / ngx) * kWalkRows;
-  const bool inT = (i < nx);
 
   double* const* Uin = (kp.sweep == 1) ? f.UL : f.U1;
   const double dtn = f.sc->dt[kp.cur], dt1 = f.sc->dt[kp.nxt];
@@ -69,27 +68,6 @@ __device__ __forceinline__ void sweepWalkYBody(const KPT<R>& kp, unsigned b) {
   const bool pb = PP && o.methode == 2;
   const bool formed = o.order > 1;
   const R* uy = f.uy[kp.nxt];
-
-  // row r of this lane's column: Phi, centroid, width, flags
-  auto loadRow = [&](long r, T* ph, double& x, T& h, int& flg) {
-    const long cb = r * nx + i;
-    double u[NC];
-    const int nload = PP ? 3 * M : NC;
-#pragma unroll
-    for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? at(Uin[q], cb) : 0.0;
-    cellPhi<M, PP, T>(u, PP ? 1.0 : at(f.VL, cb), PP ? at(f.qL, cb) : 0.0, ph);
-    x = at(f.YcL, cb);
-    h = needH ? T(widthY(kp, dt1, r, i)) : T(0.0);
-    flg = 0;
-    if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(at(f.mixte, cb), at(f.pure, cb)) : packedFlags<M, T>(ph);
-  };
-  // normal velocity of face r, between rows r-1 and r (PrepareRemap.cc:50-62)
-  auto fnv = [&](long r) {
-    const long n1 = r * (nx + 1) + i, n2 = n1 + 1;
-    return 0.0 + ((0.0 + (double)at(uy, n1)) + (double)at(uy, n2)) * 0.5;
-  };
-  // Euler length of face r (Init.cc:528-533)
-  auto faceLen = [&](long r) { return d2norm(d2sub(eulerNode(o, r, i + 1), eulerNode(o, r, i))); };
 
   // rolling state: A = row r-2, B = row r-1
   T phA[NC], phB[NC], dMB[NC], gA[NC], avA[NC], faceA[NC], dualA[M];
@@ -102,6 +80,33 @@ __device__ __forceinline__ void sweepWalkYBody(const KPT<R>& kp, unsigned b) {
   int flA = 0, flB = 0;
 
   for (int k = 0; k < kWalkRows + 4; ++k) {
+    // the column and what derives from it (addresses, Euler coordinates) are
+    // formed again for every row from an opaque copy of the index: hoisted
+    // out of the loop they would hold registers through every row
+    int iop = (int)i0;
+    asm volatile("" : "+v"(iop));
+    const long i = iop;
+    const bool inT = (i < nx);
+    // row r of this lane's column: Phi, centroid, width, flags
+    auto loadRow = [&](long r, T* ph, double& x, T& h, int& flg) {
+      const long cb = r * nx + i;
+      double u[NC];
+      const int nload = PP ? 3 * M : NC;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) u[q] = (q < nload) ? at(Uin[q], cb) : 0.0;
+      cellPhi<M, PP, T>(u, PP ? 1.0 : at(f.VL, cb), PP ? at(f.qL, cb) : 0.0, ph);
+      x = at(f.YcL, cb);
+      h = needH ? T(widthY(kp, dt1, r, i)) : T(0.0);
+      flg = 0;
+      if (mixteOpt) flg = (kp.sweep == 1) ? packFlags(at(f.mixte, cb), at(f.pure, cb)) : packedFlags<M, T>(ph);
+    };
+    // normal velocity of face r, between rows r-1 and r (PrepareRemap.cc:50-62)
+    auto fnv = [&](long r) {
+      const long n1 = r * (nx + 1) + i, n2 = n1 + 1;
+      return 0.0 + ((0.0 + (double)at(uy, n1)) + (double)at(uy, n2)) * 0.5;
+    };
+    // Euler length of face r (Init.cc:528-533)
+    auto faceLen = [&](long r) { return d2norm(d2sub(eulerNode(o, r, i + 1), eulerNode(o, r, i))); };
     const long r = a0 - 2 + k, rB = r - 1, rA = r - 2;
     // new row N
     T phN[NC];
@@ -207,8 +212,14 @@ __device__ __forceinline__ void sweepWalkYBody(const KPT<R>& kp, unsigned b) {
   }
 }
 
+// waves per SIMD the walker is compiled for: 4 for one material with limited
+// slopes (128 VGPRs, a few spilled; unbounded it takes 136 and runs 3 waves)
 template <class R, int M, bool PP>
-__global__ __launch_bounds__(256) void k_sweep_walk(KPT<R> kp) {
+constexpr int kWalkWaves = (M == 1 && !PP) ? 4 : 1;
+
+template <class R, int M, bool PP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves<R, M, PP>))) void k_sweep_walk(
+    KPT<R> kp) {
   sweepWalkYBody<R, M, PP>(kp, blockIdx.x);
 }
 
