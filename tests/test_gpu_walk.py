@@ -101,3 +101,51 @@ def test_batched_walker_equals_single_wave_strip(precision):
         for s in singles:
             s.close()
         batch.close()
+
+
+@contextmanager
+def lag_y_separate():
+    old = os.environ.get("MAHYCO_LAG_Y")
+    os.environ["MAHYCO_LAG_Y"] = "separate"
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["MAHYCO_LAG_Y"]
+        else:
+            os.environ["MAHYCO_LAG_Y"] = old
+
+
+STORE_CASES = {k: c for k, c in CASES.items() if c.nbmat == 1 and c.pp != 1}
+STORE_CASES["C3_37x301"] = _small(configs.point_explosion(), 37, 301)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name", list(STORE_CASES))
+def test_first_sweep_walker_equals_wave_strip(name, precision):
+    """With the Lagrange cell update as its own kernel on the y-first steps
+    (the tuner's choice on the 4096^2 point explosion), their first sweep
+    along y runs on the walker too, reading the stored remap vector: bitwise
+    the wave strip's state."""
+    cfg = STORE_CASES[name]
+    cfg.precision = precision
+    try:
+        log(f"[walk-first] {name} {cfg.nx}x{cfg.ny} precision={precision}")
+        with lag_y_separate():
+            with sweep_y("wave"):
+                wave = mahyco_amd.Context(cfg)
+            with sweep_y("walk"):
+                walk = mahyco_amd.Context(cfg)
+        try:
+            for chunk in (1, 1, STEPS - 2):
+                wave.step(chunk)
+                walk.step(chunk)
+                diff = _differing(walk, wave, cfg)
+                assert not diff, f"{name}: walker differs in {diff}"
+                ta, tb = walk.time_state(), wave.time_state()
+                assert (ta.t_n, ta.deltat_n) == (tb.t_n, tb.deltat_n)
+        finally:
+            wave.close()
+            walk.close()
+    finally:
+        cfg.precision = 0
